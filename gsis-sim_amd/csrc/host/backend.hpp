@@ -1,0 +1,37 @@
+#pragma once
+// The kinetic backend method set (the reference's class KineticBackend,
+// src/kinetic/kinetic_backend.hpp:59-100), host-side and free of HIP types.
+
+#include <cstddef>
+#include <memory>
+
+#include "../../../include/gsis_kinetic.h"
+
+namespace gsk {
+
+class Backend {
+ public:
+  virtual ~Backend() = default;
+  virtual const char* name() const = 0;
+  virtual int chunk_width() const = 0;
+  virtual std::size_t memory_bytes() const = 0;
+  virtual void set_equilibrium(const double* states, std::size_t n) = 0;
+  virtual void set_distribution(const double* f, std::size_t n) = 0;
+  virtual void get_distribution(double* f, std::size_t n) = 0;
+  virtual void compute_moments(double* out, std::size_t n) = 0;
+  virtual void scale_distribution(double s) = 0;
+  virtual void update_cell_params() = 0;
+  virtual void set_inflow(const gsk_inflow_state* inflow, std::size_t n) = 0;
+  virtual void boundary_sums(double* out, std::size_t n) = 0;
+  virtual void sweep(double* residual_cell, std::size_t n) = 0;
+  virtual long long gsis_correction(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
+                                    std::size_t n_flags) = 0;
+  virtual void internal_face_mass_flux(const int* cells, const int* entries, std::size_t n, double* out) = 0;
+  virtual gsk_timings take_timings() = 0;
+  virtual gsk_deq_counts deq_counts() = 0;
+};
+
+// HIP backend on the current device (FP64 or FP32 f per view.precision_fp64)
+std::unique_ptr<Backend> make_hip_backend(const gsk_setup_view& view);
+
+}  // namespace gsk

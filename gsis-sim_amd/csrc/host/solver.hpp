@@ -1,0 +1,71 @@
+#pragma once
+// CIS orchestration over a kinetic backend: moments -> closed-domain scaling -> epsilon ->
+// cell params -> pressure-inflow / wall-density updates -> boundary sums -> sweeps.
+// Restates class KineticSolver (src/kinetic/kinetic_solver.hpp:61-115,
+// src/kinetic/kinetic_solver.cpp:27-722) on top of gsk::Backend.
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "backend.hpp"
+#include "setup.hpp"
+
+namespace gsk {
+
+class KineticSolver {
+ public:
+  // takes ownership of the backend; the setup must outlive the solver
+  KineticSolver(const KineticSetup& setup, std::unique_ptr<Backend> backend);
+
+  const KineticSetup& setup() const { return s_; }
+  Backend& backend() { return *backend_; }
+
+  void initialize_uniform(double rho, const Vec3& u, double T);
+  void initialize_equilibrium(const double* states_original, std::size_t n);  // 5 per cell, original numbering
+  const gsk_iteration_info& prepare();
+  void update_mass_flow_pressure();
+  const gsk_iteration_info& sweep(bool update_pressure);
+  long long iteration() const { return iteration_; }
+  const std::vector<double>& patch_mass_flow() const { return patch_mass_flow_; }
+
+  void moments(double* out, std::size_t n);  // original numbering
+  void boundary_fluxes(double* fp, double* fm, double* fs, std::size_t nb);
+  long long apply_gsis_correction(const double* w_half, const double* w_new, std::size_t n);
+  void distribution(double* f, std::size_t n);
+  void set_distribution(const double* f, std::size_t n);
+  void residual_per_cell(double* out, std::size_t n) const;
+
+ private:
+  void correct_inflow(gsk_inflow_state& in) const;
+  void init_static_inflow();
+  void reset_after_initialization();
+  void init_wall_scale_from_cells();
+  double cell_value(const std::vector<double>& m, Index c, int k) const {
+    return m[static_cast<std::size_t>(c) * 15 + static_cast<std::size_t>(k)];
+  }
+  void update_pressure_inflow();
+  std::array<double, 10> compute_wall_unit_sums(Index b) const;
+  void process_boundary_sums();
+  void compute_epsilon();
+  void refresh_wall_inflow();
+
+  const KineticSetup& s_;
+  std::vector<PatchCondition> patches_;  // mutable copy (mass-flow inlets adjust p)
+  std::unique_ptr<Backend> backend_;
+  long long iteration_ = 0, version_ = 0, prepared_version_ = -1;
+  gsk_iteration_info info_{};
+  std::vector<double> patch_mass_flow_;
+  std::vector<double> moments_, prev_moments_;
+  bool has_prev_ = false;
+  std::vector<double> bsums_;
+  std::vector<gsk_inflow_state> inflow_;
+  std::vector<std::array<double, 10>> wall_unit_moments_;
+  std::vector<std::array<double, 5>> F_plus_, F_minus_, face_state_;
+  bool has_face_state_ = false;
+  std::vector<double> residual_cell_, inner_residual_;
+  double mass_target_ = 0.0;
+  bool has_mass_target_ = false;
+};
+
+}  // namespace gsk

@@ -1,0 +1,1118 @@
+// HIP backend of the kinetic step for gfx950 (MI355X): kernels and the host-side backend object.
+//
+// Layout in HBM: f[cell][storage position] (velocities contiguous, so the 64 lanes of a wave read
+// 64 consecutive velocities of one cell: coalesced), chunk workspaces [cell][chunk width], per-cell
+// tables as arrays of structs read wave-uniformly (scalar loads). Per-cell reductions over
+// velocities (moments, discrete-equilibrium Newton sums, boundary sums) run one workgroup per cell
+// or face with FP64 accumulation and a fixed-order wave + LDS tree, so results are deterministic
+// and independent of launch splitting (reference rule: docs/design_cis_gpu.md:121-123).
+//
+// Reference kernels replaced (src/kinetic/kinetic_cuda_kernels.cu): k_set_equilibrium (:147),
+// k_moments (:171), k_scale (:207), k_cell_params / k_cell_params_deq (:221-260),
+// k_maxwell_reflection (:262), k_boundary_sums (:274), k_internal_face_mass_flux (:326),
+// k_gradient (:343), k_residual + reduce (:357-381), k_sweep_forward / backward (:383-425),
+// k_gsis_correction (:427). Host flow: src/kinetic/kinetic_backend_cuda.cpp:89-509.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../host/backend.hpp"
+#include "../host/core.hpp"
+#include "kinetic_view.hpp"
+
+namespace gsk {
+namespace dev {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// ---- fixed-order block reductions (wave butterfly-free: lane 0 of each wave, then waves in order) ----
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) lds[w * N + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < kWaves; ++i) s += lds[i * N + k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+template <class F, bool Max>
+__device__ __forceinline__ F block_extreme(F v, F* lds) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const F o = __shfl_down(v, off, 64);
+    v = Max ? dmax(v, o) : dmin(v, o);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  F r = lds[0];
+  for (int i = 1; i < kWaves; ++i) r = Max ? dmax(r, lds[i]) : dmin(r, lds[i]);
+  __syncthreads();
+  return r;
+}
+
+// Discrete conservative Maxwellian of one cell, all threads of the block cooperating on the 20
+// sums of each Newton step; every thread ends with the same result (discrete_maxwell_seq,
+// kinetic_kernels.hpp:499-540). Returns iterations, 0 when resolved, -1 on failure.
+template <class F>
+__device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, double uy, double uz, double T,
+                                      const Target<F>* poly, Maxw<F>& out, double (&gamma)[5], double* lds) {
+  for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+  out = maxw_from_state<F>(rho, ux, uy, uz, T);
+  if (deq_resolved(v.deq, ux, uy, uz, T)) return 0;
+  double beta[5];
+  deq_initial_beta(rho, T, beta);
+  const F fux = static_cast<F>(ux), fuy = static_cast<F>(uy), fuz = static_cast<F>(uz);
+  double rel_prev = 1e300;
+  for (int it = 0; it < v.deq.max_iterations; ++it) {
+    const Maxw<F> m = deq_params<F>(beta, ux, uy, uz);
+    double sums[kDeqSums];
+    for (int k = 0; k < kDeqSums; ++k) sums[k] = 0.0;
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+      const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+      const F cx = x - fux, cy = y - fuy, cz = z - fuz;
+      const F e = maxw_eval(m, x, y, z);
+      const F pm = poly != nullptr ? shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz) : F(0);
+      F t[kDeqSums];
+      deq_terms<F>(e, pm, cx, cy, cz, t);
+      for (int k = 0; k < kDeqSums; ++k) sums[k] += static_cast<double>(t[k]);
+    }
+    block_sum<kDeqSums>(sums, lds);
+    double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
+    double rel = 0.0;
+    const int r = deq_newton_step(sums, v.weight_d, rho, T, v.deq.tolerance, b, rel);
+    if (deq_accept(r, rel, rel_prev, it + 1 == v.deq.max_iterations)) {
+      out = m;
+      if (poly != nullptr && !deq_shakhov_correction(sums, v.weight_d, gamma))
+        for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+      return it + 1;
+    }
+    if (r < 0) break;
+    for (int k = 0; k < 5; ++k) beta[k] = b[k];
+    rel_prev = rel;
+  }
+  out = maxw_from_state<F>(rho, ux, uy, uz, T);
+  for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+  return -1;
+}
+
+template <class F>
+__device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, double uy, double uz, double T, int* result,
+                                      double* lds) {
+  if (result) *result = 0;
+  if (!v.discrete_eq) return maxw_from_state<F>(rho, ux, uy, uz, T);
+  Maxw<F> m;
+  double gamma[5];
+  const int r = block_discrete_maxwell<F>(v, rho, ux, uy, uz, T, nullptr, m, gamma, lds);
+  if (result) *result = r;
+  return m;
+}
+
+// ---- kernels ------------------------------------------------------------------------------
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_set_equilibrium(View<F> v, const double* __restrict__ states,
+                                                            double* __restrict__ shift) {
+  __shared__ double lds[kWaves * kDeqSums];
+  for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
+    const double* st = states + static_cast<i64>(c) * 5;
+    const Maxw<F> m = equilibrium_params<F>(v, st[0], st[1], st[2], st[3], st[4], nullptr, lds);
+    F* fc = v.f + static_cast<i64>(c) * v.nv;
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) fc[p] = maxw_eval(m, v.xi_x[p], v.xi_y[p], v.xi_z[p]);
+    if (threadIdx.x == 0) {
+      shift[c * 3 + 0] = st[1];
+      shift[c * 3 + 1] = st[2];
+      shift[c * 3 + 2] = st[3];
+    }
+  }
+}
+
+// 13 shifted sums per cell in FP64 -> 15 moments; updates the shift (k_moments)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_moments(View<F> v, const double* __restrict__ xd, const double* __restrict__ yd,
+                                                    const double* __restrict__ zd, double* __restrict__ shift,
+                                                    double* __restrict__ moments) {
+  __shared__ double lds[kWaves * kMomentSums];
+  for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
+    const double u0x = shift[c * 3 + 0], u0y = shift[c * 3 + 1], u0z = shift[c * 3 + 2];
+    double s[kMomentSums];
+    for (int k = 0; k < kMomentSums; ++k) s[k] = 0.0;
+    const F* fc = v.f + static_cast<i64>(c) * v.nv;
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+      double t[kMomentSums];
+      moment_terms<double>(static_cast<double>(fc[p]), xd[p] - u0x, yd[p] - u0y, zd[p] - u0z, t);
+      for (int k = 0; k < kMomentSums; ++k) s[k] += t[k];
+    }
+    block_sum<kMomentSums>(s, lds);
+    if (threadIdx.x == 0) {
+      double* m = moments + static_cast<i64>(c) * kMoments;
+      finalize_moments(s, v.weight_d, u0x, u0y, u0z, m);
+      if (m[0] > 0.0) {
+        shift[c * 3 + 0] = m[1];
+        shift[c * 3 + 1] = m[2];
+        shift[c * 3 + 2] = m[3];
+      }
+    }
+  }
+}
+
+template <class F>
+__global__ void k_scale_f(F* __restrict__ f, i64 n, F s) {
+  for (i64 i = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<i64>(gridDim.x) * blockDim.x)
+    f[i] *= s;
+}
+
+__global__ void k_scale_moments(double* __restrict__ moments, int n_cells, double s) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  double* m = moments + static_cast<i64>(c) * kMoments;
+  m[0] *= s;
+  for (int k = 5; k < kMoments; ++k) m[k] *= s;
+}
+
+// Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
+// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double* __restrict__ moments,
+                                                        const double* __restrict__ volume, const double* __restrict__ vdt,
+                                                        double mu0, double omega, double Pr, CellP<F>* __restrict__ cells,
+                                                        int* __restrict__ flags) {
+  __shared__ double lds[kWaves * kDeqSums];
+  for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
+    const double* m = moments + static_cast<i64>(c) * kMoments;
+    CellP<F> cp = make_cell_params<F>(m, volume[c], vdt[c], mu0, omega, Pr, v.cold_bgk);
+    int r = 0;
+    if (v.discrete_eq) {
+      Maxw<F> eq;
+      double gamma[5];
+      r = block_discrete_maxwell<F>(v, m[0], m[1], m[2], m[3], m[4], &cp.s, eq, gamma, lds);
+      if (r > 0) {
+        cp.s.eq = eq;
+        cp.s.g0 = static_cast<F>(gamma[0]);
+        cp.s.gx = static_cast<F>(gamma[1]);
+        cp.s.gy = static_cast<F>(gamma[2]);
+        cp.s.gz = static_cast<F>(gamma[3]);
+        cp.s.g4 = static_cast<F>(gamma[4]);
+      } else if (r < 0 && v.deq_fallback) {
+        cp.coll = F(0);
+        cp.diag = static_cast<F>(vdt[c]);
+      }
+    }
+    if (threadIdx.x == 0) {
+      cells[c] = cp;
+      flags[c] = r;
+    }
+  }
+}
+
+// specular caches of partially accommodating walls (k_maxwell_reflection)
+template <class F, bool Reconstruct>
+__global__ void k_maxwell_reflection(View<F> v, F* __restrict__ out) {
+  const int b = blockIdx.y;
+  if (b >= v.n_bfaces || v.mw_off[b] < 0) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.nv) return;
+  const BFace<F>& bf = v.bf[b];
+  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  out[v.mw_off[b] + p] = (x * bf.nx + y * bf.ny + z * bf.nz) < F(0) ? reflected_value<Reconstruct>(v, bf, p, x, y, z) : F(0);
+}
+
+// 20 half-space sums per boundary face in FP64 (k_boundary_sums). faces: optional subset list.
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_boundary_sums(View<F> v, const int* __restrict__ faces, int n_faces,
+                                                          double* __restrict__ out) {
+  __shared__ double lds[kWaves * kBoundarySums];
+  for (int i = blockIdx.x; i < n_faces; i += gridDim.x) {
+    const int b = faces ? faces[i] : i;
+    const BFace<F> bf = v.bf[b];
+    double acc[kBoundarySums];
+    for (int k = 0; k < kBoundarySums; ++k) acc[k] = 0.0;
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+      const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+      const F an = x * bf.nx + y * bf.ny + z * bf.nz;
+      F fb;
+      int base;
+      if (an >= F(0) || bf.kind == kBcPeriodic) {
+        fb = reconstruct_at_boundary(v, bf, p, x, y, z);
+        base = 0;
+      } else {
+        if (bf.kind == kBcSymmetry) {
+          const int pm = p ^ bf.mirror_xor;
+          fb = reconstruct_at_boundary(v, bf, pm, v.xi_x[pm], v.xi_y[pm], v.xi_z[pm]);
+        } else if (bf.kind == kBcMaxwell) {
+          fb = maxwell_inflow(v, b, p, x, y, z);
+        } else {
+          fb = maxw_eval(v.inflow[b].m, x, y, z);
+        }
+        base = 5;
+      }
+      const F wf = v.weight * fb;
+      const F e = F(0.5) * (x * x + y * y + z * z);
+      const F af = an * wf;
+      const double tf[5] = {static_cast<double>(af), static_cast<double>(af * x), static_cast<double>(af * y),
+                            static_cast<double>(af * z), static_cast<double>(af * e)};
+      const double tm[5] = {static_cast<double>(wf), static_cast<double>(wf * x), static_cast<double>(wf * y),
+                            static_cast<double>(wf * z), static_cast<double>(wf * e)};
+      // branch-free accumulation into the outgoing (0..4, 10..14) or incoming (5..9, 15..19) slots
+      for (int k = 0; k < 5; ++k) {
+        acc[k] += base == 0 ? tf[k] : 0.0;
+        acc[5 + k] += base == 5 ? tf[k] : 0.0;
+        acc[10 + k] += base == 0 ? tm[k] : 0.0;
+        acc[15 + k] += base == 5 ? tm[k] : 0.0;
+      }
+    }
+    block_sum<kBoundarySums>(acc, lds);
+    if (threadIdx.x < kBoundarySums) out[static_cast<i64>(b) * kBoundarySums + threadIdx.x] = acc[threadIdx.x];
+  }
+}
+
+// section mass flux through listed internal faces (k_internal_face_mass_flux)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int* __restrict__ cells,
+                                                           const int* __restrict__ entries, int n, double* __restrict__ out) {
+  __shared__ double lds[kWaves];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int c = cells[i];
+    const CsrE<F> e = v.csr[entries[i]];
+    double acc[1] = {0.0};
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x)
+      acc[0] += static_cast<double>(internal_face_flux_term(v, c, e, p, v.xi_x[p], v.xi_y[p], v.xi_z[p]));
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) out[i] = static_cast<double>(v.weight) * acc[0];
+  }
+}
+
+// XCD-aware (cell, velocity tile) mapping: workgroups are dealt round-robin to the 8 XCDs, so
+// give XCD x a contiguous range of cells; neighbouring cells (close in the RCB/RCM numbering)
+// then share one L2 when their f rows are re-read as upwind / stencil values.
+__device__ __forceinline__ bool xcd_tile(i64 n_items, int tiles, int& cell, int& tile) {
+  const i64 bid = blockIdx.x;
+  const i64 per = (static_cast<i64>(gridDim.x) + 7) / 8;
+  const i64 logical = (bid % 8) * per + bid / 8;
+  if (logical >= n_items) return false;
+  cell = static_cast<int>(logical / tiles);
+  tile = static_cast<int>(logical % tiles);
+  return true;
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles) {
+  int c, tile;
+  if (!xcd_tile(static_cast<i64>(v.n_cells) * tiles, tiles, c, tile)) return;
+  const int t = tile * kBlock + threadIdx.x;
+  if (t >= ch.kc) return;
+  const int p = ch.k0 + t;
+  F gx, gy, gz;
+  limited_gradient(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
+  const i64 wi = static_cast<i64>(c) * ch.stride + t;
+  ch.gx[wi] = gx;
+  ch.gy[wi] = gy;
+  ch.gz[wi] = gz;
+}
+
+// residual R into rres and per (cell, tile) partial sums of R^2 (k_residual)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int tiles, F* __restrict__ rres,
+                                                     double* __restrict__ partial) {
+  __shared__ double lds[kWaves];
+  int c, tile;
+  if (!xcd_tile(static_cast<i64>(v.n_cells) * tiles, tiles, c, tile)) return;
+  const int t = tile * kBlock + threadIdx.x;
+  double r2[1] = {0.0};
+  if (t < ch.kc) {
+    const F R = residual_value(v, ch, c, t);
+    rres[static_cast<i64>(c) * ch.stride + t] = R;
+    r2[0] = static_cast<double>(R) * static_cast<double>(R);
+  }
+  block_sum<1>(r2, lds);
+  if (threadIdx.x == 0) partial[static_cast<i64>(c) * tiles + tile] = r2[0];
+}
+
+__global__ void k_residual_reduce(const double* __restrict__ partial, int tiles, int n_cells, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  double s = 0.0;
+  for (int i = 0; i < tiles; ++i) s += partial[static_cast<i64>(c) * tiles + i];
+  out[c] += s;
+}
+
+// LU-SGS forward sweep: one thread per (spatial block, chunk velocity) walks the block's cells in
+// its direction's order (sweep_forward_cell, kinetic_kernels.hpp:290-324). cur holds dF* of this
+// pass, prev the previous pass' dF (nullptr on the first pass: beta diagonal compensation).
+template <class F>
+__global__ void k_sweep_forward(View<F> v, int k0, int kc, int stride, int n_blocks, const int* __restrict__ block_off,
+                                const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
+  const i64 lane = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x;
+  if (lane >= static_cast<i64>(n_blocks) * kc) return;
+  const int b = static_cast<int>(lane / kc), t = static_cast<int>(lane % kc);
+  const int p = k0 + t;
+  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  const int c0 = block_off[b], c1 = block_off[b + 1];
+  const int* order = nullptr;
+  const int* pos = nullptr;
+  if (v.n_sweep_dirs > 0) {
+    const i64 off = static_cast<i64>(v.sweep_dir_of[p]) * v.n_cells;
+    order = v.sweep_order + off;
+    pos = v.sweep_pos + off;
+  }
+  for (int k = c0; k < c1; ++k) {
+    const int cell = order ? order[k] : k;
+    F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
+    const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+    const int pi = pos ? pos[cell] : cell;
+    for (int q = e0; q < e1; ++q) {
+      const CsrE<F>& e = v.csr[q];
+      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+      const F a = x * e.sx + y * e.sy + z * e.sz;
+      if (a > F(0)) {
+        ap += a;
+      } else if (e.other >= 0) {
+        if (e.other < c0 || e.other >= c1) {
+          if (prev != nullptr)
+            Lout += a * prev[static_cast<i64>(e.other) * stride + t];
+          else
+            am_out -= a;
+        } else if ((pos ? pos[e.other] : e.other) < pi) {
+          L += a * cur[static_cast<i64>(e.other) * stride + t];
+        }
+      }
+    }
+    const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+    const i64 wi = static_cast<i64>(cell) * stride + t;
+    cur[wi] = (-rres[wi] - Lout - L) / D;
+  }
+}
+
+// LU-SGS backward sweep (sweep_backward_cell, kinetic_kernels.hpp:328-358); f += dF when update_f
+template <class F>
+__global__ void k_sweep_backward(View<F> v, int k0, int kc, int stride, int n_blocks, const int* __restrict__ block_off,
+                                 F* cur, int has_prev, int update_f) {
+  const i64 lane = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x;
+  if (lane >= static_cast<i64>(n_blocks) * kc) return;
+  const int b = static_cast<int>(lane / kc), t = static_cast<int>(lane % kc);
+  const int p = k0 + t;
+  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  const int c0 = block_off[b], c1 = block_off[b + 1];
+  const int* order = nullptr;
+  const int* pos = nullptr;
+  if (v.n_sweep_dirs > 0) {
+    const i64 off = static_cast<i64>(v.sweep_dir_of[p]) * v.n_cells;
+    order = v.sweep_order + off;
+    pos = v.sweep_pos + off;
+  }
+  for (int k = c1 - 1; k >= c0; --k) {
+    const int cell = order ? order[k] : k;
+    F ap = F(0), am_out = F(0), U = F(0);
+    const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+    const int pi = pos ? pos[cell] : cell;
+    for (int q = e0; q < e1; ++q) {
+      const CsrE<F>& e = v.csr[q];
+      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+      const F a = x * e.sx + y * e.sy + z * e.sz;
+      if (a > F(0)) {
+        ap += a;
+      } else if (e.other >= 0) {
+        if (e.other < c0 || e.other >= c1) {
+          if (!has_prev) am_out -= a;
+        } else if ((pos ? pos[e.other] : e.other) > pi) {
+          U += a * cur[static_cast<i64>(e.other) * stride + t];
+        }
+      }
+    }
+    const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+    const i64 wi = static_cast<i64>(cell) * stride + t;
+    const F df = cur[wi] - U / D;
+    cur[wi] = df;
+    if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
+  }
+}
+
+// GSIS correction f <- f - M(W_half) + M(W_new) with the negative-value rules (k_gsis_correction)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_gsis_correction(View<F> v, const double* __restrict__ w_half,
+                                                            const double* __restrict__ w_new, int mode,
+                                                            double* __restrict__ shift, int* __restrict__ flags) {
+  __shared__ double lds[kWaves * kDeqSums];
+  __shared__ F ldsf[kWaves];
+  for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
+    const double* wh = w_half + static_cast<i64>(c) * 5;
+    const double* wn = w_new + static_cast<i64>(c) * 5;
+    int rh = 0, rn = 0;
+    const Maxw<F> mh = equilibrium_params<F>(v, wh[0], wh[1], wh[2], wh[3], wh[4], &rh, lds);
+    const Maxw<F> mn = equilibrium_params<F>(v, wn[0], wn[1], wn[2], wn[3], wn[4], &rn, lds);
+    if (v.deq_fallback && (rh < 0 || rn < 0)) {
+      if (threadIdx.x == 0) {
+        flags[c] = 2;
+        shift[c * 3 + 0] = wh[1];
+        shift[c * 3 + 1] = wh[2];
+        shift[c * 3 + 2] = wh[3];
+      }
+      continue;
+    }
+    F* fc = v.f + static_cast<i64>(c) * v.nv;
+    const double threshold = -1e-8 * wn[0] / (wn[4] * ::sqrt(wn[4]));
+    const F thr = static_cast<F>(threshold);
+    bool replace = false;
+    if (mode == 2) {
+      F alpha = F(1);
+      for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+        const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        alpha = dmin(alpha, correction_alpha_limit(fc[p], maxw_eval(mn, x, y, z) - maxw_eval(mh, x, y, z), thr));
+      }
+      alpha = block_extreme<F, false>(alpha, ldsf);
+      for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+        const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        fc[p] += alpha * (maxw_eval(mn, x, y, z) - maxw_eval(mh, x, y, z));
+      }
+      replace = alpha < F(1);
+    } else if (mode == 3) {
+      F theta = F(0);
+      for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+        const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        const F e = maxw_eval(mn, x, y, z);
+        theta = dmax(theta, blend_theta(fc[p], e - maxw_eval(mh, x, y, z), e, thr));
+      }
+      theta = block_extreme<F, true>(theta, ldsf);
+      for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+        const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        const F e = maxw_eval(mn, x, y, z);
+        const F g = fc[p] + (e - maxw_eval(mh, x, y, z));
+        fc[p] = (F(1) - theta) * g + theta * e;
+      }
+      replace = theta > F(0);
+    } else {
+      F fmin = F(1e300 > 1e38 ? 3.0e38 : 0);
+      bool any = false;
+      for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+        const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        const F old = fc[p];
+        const F xx = old + (maxw_eval(mn, x, y, z) - maxw_eval(mh, x, y, z));
+        fc[p] = xx;
+        const F chk = negative_check_value(old, xx, mode == 1);
+        fmin = any ? dmin(fmin, chk) : chk;
+        any = true;
+      }
+      if (!any) fmin = static_cast<F>(3.0e38);
+      fmin = block_extreme<F, false>(fmin, ldsf);
+      replace = static_cast<double>(fmin) < threshold;
+      if (replace)
+        for (int p = threadIdx.x; p < v.nv; p += blockDim.x) fc[p] = maxw_eval(mn, v.xi_x[p], v.xi_y[p], v.xi_z[p]);
+    }
+    if (threadIdx.x == 0) {
+      flags[c] = replace ? 1 : 0;
+      shift[c * 3 + 0] = wn[1];
+      shift[c * 3 + 1] = wn[2];
+      shift[c * 3 + 2] = wn[3];
+    }
+    __syncthreads();
+  }
+}
+
+template <class F>
+__global__ void k_from_double(const double* __restrict__ src, F* __restrict__ dst, i64 n) {
+  for (i64 i = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<i64>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<F>(src[i]);
+}
+template <class F>
+__global__ void k_to_double(const F* __restrict__ src, double* __restrict__ dst, i64 n) {
+  for (i64 i = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<i64>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<double>(src[i]);
+}
+
+}  // namespace dev
+
+// ==============================================================================================
+// Host-side backend
+// ==============================================================================================
+namespace {
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) fail_at(file, line, strf("HIP error in %s: %s", what, hipGetErrorString(e)));
+}
+#define HIPCHK(x) hip_check((x), #x, __FILE__, __LINE__)
+
+// owning device buffer
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  std::size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(std::size_t count) {
+    release();
+    n = count;
+    if (count == 0) return;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw Error(strf("device allocation of %zu bytes failed (%s)", count * sizeof(T), hipGetErrorString(e)));
+    }
+  }
+  void upload(const T* src, std::size_t count, hipStream_t s) {
+    if (count != n) alloc(count);
+    if (count) HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  std::size_t bytes() const { return n * sizeof(T); }
+};
+
+enum Phase { kMoments, kBoundary, kGradient, kResidual, kForward, kBackward, kOther, kNumPhases };
+
+template <class F>
+class HipBackend final : public Backend {
+ public:
+  explicit HipBackend(const gsk_setup_view& s) {
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    nc_ = s.n_cells;
+    nv_ = s.nv;
+    nb_ = s.n_bfaces;
+    n_blocks_ = s.n_blocks;
+    block_passes_ = s.block_passes;
+    negative_mode_ = s.negative_check;
+    mu0_ = s.mu0;
+    omega_ = s.omega;
+    Pr_ = s.Pr;
+    GSK_CHECK(nc_ > 0 && nv_ > 0 && n_blocks_ >= 1, "setup view is empty");
+    GSK_CHECK(s.orbit_size >= 1 && nv_ % s.orbit_size == 0, "velocity count is not a multiple of the orbit size");
+    // velocity tables
+    std::vector<F> xf(static_cast<std::size_t>(nv_)), yf(xf.size()), zf(xf.size());
+    for (int p = 0; p < nv_; ++p) {
+      xf[static_cast<std::size_t>(p)] = static_cast<F>(s.xi_x[p]);
+      yf[static_cast<std::size_t>(p)] = static_cast<F>(s.xi_y[p]);
+      zf[static_cast<std::size_t>(p)] = static_cast<F>(s.xi_z[p]);
+    }
+    xi_x_.upload(xf.data(), xf.size(), stream_);
+    xi_y_.upload(yf.data(), yf.size(), stream_);
+    xi_z_.upload(zf.data(), zf.size(), stream_);
+    xd_.upload(s.xi_x, static_cast<std::size_t>(nv_), stream_);
+    yd_.upload(s.xi_y, static_cast<std::size_t>(nv_), stream_);
+    zd_.upload(s.xi_z, static_cast<std::size_t>(nv_), stream_);
+    // mesh tables
+    const int n_entries = s.csr_offsets[nc_];
+    csr_off_.upload(s.csr_offsets, static_cast<std::size_t>(nc_) + 1, stream_);
+    std::vector<dev::CsrE<F>> csr(static_cast<std::size_t>(n_entries));
+    for (int k = 0; k < n_entries; ++k) {
+      const gsk_csr_entry& e = s.csr[k];
+      dev::CsrE<F>& o = csr[static_cast<std::size_t>(k)];
+      o.other = e.other;
+      o.bface = e.bface;
+      o.sx = static_cast<F>(e.s[0]);
+      o.sy = static_cast<F>(e.s[1]);
+      o.sz = static_cast<F>(e.s[2]);
+      o.dsx = static_cast<F>(e.ds[0]);
+      o.dsy = static_cast<F>(e.ds[1]);
+      o.dsz = static_cast<F>(e.ds[2]);
+      o.dox = static_cast<F>(e.dother[0]);
+      o.doy = static_cast<F>(e.dother[1]);
+      o.doz = static_cast<F>(e.dother[2]);
+      o.cx = static_cast<F>(e.c[0]);
+      o.cy = static_cast<F>(e.c[1]);
+      o.cz = static_cast<F>(e.c[2]);
+      GSK_CHECK((e.other >= 0 && e.other < nc_) || (e.other < 0 && e.bface >= 0 && e.bface < nb_),
+                "CSR entry refers to a cell or face out of range");
+    }
+    csr_.upload(csr.data(), csr.size(), stream_);
+    std::vector<F> eps2(static_cast<std::size_t>(nc_));
+    for (int c = 0; c < nc_; ++c) eps2[static_cast<std::size_t>(c)] = static_cast<F>(s.eps2[c]);
+    eps2_.upload(eps2.data(), eps2.size(), stream_);
+    volume_.upload(s.volume, static_cast<std::size_t>(nc_), stream_);
+    vdt_.upload(s.vdt, static_cast<std::size_t>(nc_), stream_);
+    std::vector<dev::BFace<F>> bf(static_cast<std::size_t>(nb_));
+    for (int b = 0; b < nb_; ++b) {
+      const gsk_bface& i = s.bfaces[b];
+      dev::BFace<F>& o = bf[static_cast<std::size_t>(b)];
+      o.owner = i.owner;
+      o.kind = i.kind;
+      o.mirror_xor = i.mirror_xor;
+      o.patch = i.patch;
+      o.nx = static_cast<F>(i.n[0]);
+      o.ny = static_cast<F>(i.n[1]);
+      o.nz = static_cast<F>(i.n[2]);
+      o.area = static_cast<F>(i.area);
+      o.dx = static_cast<F>(i.d[0]);
+      o.dy = static_cast<F>(i.d[1]);
+      o.dz = static_cast<F>(i.d[2]);
+      GSK_CHECK(i.owner >= 0 && i.owner < nc_, "boundary face owner out of range");
+      GSK_CHECK(i.mirror_xor >= 0 && i.mirror_xor < nv_, "boundary face mirror out of range");
+    }
+    bfaces_.upload(bf.data(), bf.size(), stream_);
+    host_bface_kind_.resize(static_cast<std::size_t>(nb_));
+    for (int b = 0; b < nb_; ++b) host_bface_kind_[static_cast<std::size_t>(b)] = s.bfaces[b].kind;
+    inflow_.alloc(static_cast<std::size_t>(nb_));
+    HIPCHK(hipMemsetAsync(inflow_.p, 0, inflow_.bytes(), stream_));
+    block_off_.upload(s.block_offsets, static_cast<std::size_t>(n_blocks_) + 1, stream_);
+    n_sweep_dirs_ = s.n_sweep_dirs;
+    if (n_sweep_dirs_ > 0) {
+      sweep_order_.upload(s.sweep_order, static_cast<std::size_t>(n_sweep_dirs_) * nc_, stream_);
+      sweep_pos_.upload(s.sweep_pos, static_cast<std::size_t>(n_sweep_dirs_) * nc_, stream_);
+      sweep_dir_of_.upload(s.sweep_dir_of, static_cast<std::size_t>(nv_), stream_);
+    }
+    // Maxwell walls
+    n_mw_ = s.n_maxwell_faces;
+    if (n_mw_ > 0) {
+      mw_off_.upload(reinterpret_cast<const long long*>(s.maxwell_offset), static_cast<std::size_t>(nb_), stream_);
+      std::vector<F> frac(static_cast<std::size_t>(nb_));
+      for (int b = 0; b < nb_; ++b) frac[static_cast<std::size_t>(b)] = static_cast<F>(s.maxwell_specular[b]);
+      mw_frac_.upload(frac.data(), frac.size(), stream_);
+      specular_.alloc(static_cast<std::size_t>(n_mw_) * nv_);
+      specular_first_.alloc(static_cast<std::size_t>(n_mw_) * nv_);
+      HIPCHK(hipMemsetAsync(specular_.p, 0, specular_.bytes(), stream_));
+      HIPCHK(hipMemsetAsync(specular_first_.p, 0, specular_first_.bytes(), stream_));
+      grid_to_storage_.upload(s.grid_to_storage, static_cast<std::size_t>(nv_), stream_);
+    }
+    for (int a = 0; a < 3; ++a) {
+      grid_n_[a] = s.grid_n[a];
+      deq_.lo[a] = s.deq_lo[a];
+      deq_.hi[a] = s.deq_hi[a];
+      deq_.d[a] = s.deq_d[a];
+    }
+    deq_.criterion = s.deq_criterion;
+    deq_.max_iterations = s.deq_max_iterations;
+    deq_.tolerance = s.deq_tolerance;
+    discrete_eq_ = s.discrete_eq;
+    deq_fallback_ = s.deq_fallback_collisionless;
+    cold_bgk_ = s.cold_bgk_temperature;
+    limiter_mode_ = s.limiter_mode;
+    positivity_ = s.positivity;
+    diag_scale_ = s.diag_scale;
+    interface_damping_ = s.interface_damping;
+    weight_ = s.weight;
+    // state
+    const std::size_t nf = static_cast<std::size_t>(nc_) * static_cast<std::size_t>(nv_);
+    // fit check before the large allocations (kinetic_backend_cuda.cpp:103-106)
+    std::size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    GSK_CHECK(nf * sizeof(F) < free_b, strf("f needs %.2f GB but only %.2f GB of device memory is free",
+                                            nf * sizeof(F) / 1e9, free_b / 1e9));
+    f_.alloc(nf);
+    HIPCHK(hipMemsetAsync(f_.p, 0, f_.bytes(), stream_));
+    cells_.alloc(static_cast<std::size_t>(nc_));
+    HIPCHK(hipMemsetAsync(cells_.p, 0, cells_.bytes(), stream_));
+    moments_.alloc(static_cast<std::size_t>(nc_) * dev::kMoments);
+    HIPCHK(hipMemsetAsync(moments_.p, 0, moments_.bytes(), stream_));
+    shift_.alloc(static_cast<std::size_t>(nc_) * 3);
+    HIPCHK(hipMemsetAsync(shift_.p, 0, shift_.bytes(), stream_));
+    flags_.alloc(static_cast<std::size_t>(nc_));
+    HIPCHK(hipMemsetAsync(flags_.p, 0, flags_.bytes(), stream_));
+    residual_.alloc(static_cast<std::size_t>(nc_));
+    bsums_.alloc(static_cast<std::size_t>(nb_) * dev::kBoundarySums + 1);
+    // chunk width: requested (rounded up to the orbit size), or as wide as 6 workspace arrays fit
+    const int orbit = s.orbit_size;
+    long long k = s.velocity_chunk > 0 ? s.velocity_chunk : nv_;
+    k = (k + orbit - 1) / orbit * orbit;
+    k = std::min<long long>(k, nv_);
+    if (s.velocity_chunk == 0) {
+      HIPCHK(hipMemGetInfo(&free_b, &total_b));
+      const double budget = 0.85 * static_cast<double>(free_b);
+      const double per_k = 6.0 * nc_ * sizeof(F);
+      while (k > orbit && per_k * k > budget) k = std::max<long long>(orbit, (k / 2 + orbit - 1) / orbit * orbit);
+    }
+    chunk_ = static_cast<int>(k);
+    tiles_ = (chunk_ + dev::kBlock - 1) / dev::kBlock;
+    const std::size_t work = static_cast<std::size_t>(nc_) * static_cast<std::size_t>(chunk_);
+    gx_.alloc(work);
+    gy_.alloc(work);
+    gz_.alloc(work);
+    rres_.alloc(work);
+    bufA_.alloc(work);
+    bufB_.alloc(work);
+    partial_.alloc(static_cast<std::size_t>(nc_) * tiles_);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned_), pinned_bytes(), hipHostMallocDefault));
+    for (int i = 0; i < kNumPhases; ++i) timings_[i] = 0.0;
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  ~HipBackend() override {
+    for (auto& e : events_) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    for (auto& e : free_events_) (void)hipEventDestroy(e);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+
+  const char* name() const override { return sizeof(F) == 8 ? "hip-fp64" : "hip-fp32"; }
+  int chunk_width() const override { return chunk_; }
+  std::size_t memory_bytes() const override {
+    return f_.bytes() + gx_.bytes() + gy_.bytes() + gz_.bytes() + rres_.bytes() + bufA_.bytes() + bufB_.bytes() +
+           csr_.bytes() + moments_.bytes() + specular_.bytes() + specular_first_.bytes() + partial_.bytes() +
+           cells_.bytes() + (n_sweep_dirs_ > 0 ? sweep_order_.bytes() + sweep_pos_.bytes() : 0);
+  }
+
+  void set_equilibrium(const double* states, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nc_) * 5, "states length differs from 5 x cells");
+    begin(kOther);
+    DBuf<double> st;
+    st.upload(states, n, stream_);
+    dev::k_set_equilibrium<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), st.p, shift_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void set_distribution(const double* f, std::size_t n) override {
+    GSK_CHECK(n == f_.n, "distribution length differs from cells x velocities");
+    if (sizeof(F) == 8) {
+      HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    } else {
+      convert_in(f, n);
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void get_distribution(double* f, std::size_t n) override {
+    GSK_CHECK(n == f_.n, "distribution length differs from cells x velocities");
+    if (sizeof(F) == 8) {
+      HIPCHK(hipMemcpyAsync(f, f_.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    } else {
+      convert_out(f, n);
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void compute_moments(double* out, std::size_t n) override {
+    GSK_CHECK(n == moments_.n, "moments output length differs from 15 x cells");
+    begin(kMoments);
+    dev::k_moments<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), xd_.p, yd_.p, zd_.p, shift_.p, moments_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    download(out, moments_.p, n);
+  }
+
+  void scale_distribution(double s) override {
+    begin(kOther);
+    dev::k_scale_f<F><<<4096, 256, 0, stream_>>>(f_.p, static_cast<long long>(f_.n), static_cast<F>(s));
+    dev::k_scale_moments<<<(nc_ + 255) / 256, 256, 0, stream_>>>(moments_.p, nc_, s);
+    HIPCHK(hipGetLastError());
+    end();
+  }
+
+  void update_cell_params() override {
+    begin(kOther);
+    dev::k_cell_params<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), moments_.p, volume_.p, vdt_.p, mu0_, omega_, Pr_,
+                                                                     cells_.p, flags_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    deq_flags_.resize(static_cast<std::size_t>(nc_));
+    download(deq_flags_.data(), flags_.p, deq_flags_.size());
+  }
+
+  gsk_deq_counts deq_counts() override {
+    gsk_deq_counts out{0, 0, 0};
+    if (!discrete_eq_) return out;
+    for (const int x : deq_flags_) {
+      if (x > 0) ++out.corrected;
+      if (x < 0) ++out.failed;
+      out.max_iterations = std::max(out.max_iterations, x);
+    }
+    return out;
+  }
+
+  void set_inflow(const gsk_inflow_state* in, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nb_), "inflow length differs from the boundary-face count");
+    std::vector<dev::Inflow<F>> h(n);
+    for (std::size_t b = 0; b < n; ++b) {
+      h[b].m = dev::maxw_from_state<F>(in[b].rho, in[b].ux, in[b].uy, in[b].uz, in[b].T);
+      h[b].scale = static_cast<F>(in[b].scale);
+    }
+    if (n) HIPCHK(hipMemcpyAsync(inflow_.p, h.data(), n * sizeof(dev::Inflow<F>), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void boundary_sums(double* out, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nb_) * dev::kBoundarySums, "boundary sums length differs from 20 x faces");
+    begin(kBoundary);
+    if (n_mw_ > 0) {
+      const dim3 grid((nv_ + 255) / 256, nb_);
+      dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
+      dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+    }
+    if (nb_ > 0) dev::k_boundary_sums<F><<<std::min(nb_, 65536), dev::kBlock, 0, stream_>>>(view(), nullptr, nb_, bsums_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    download(out, bsums_.p, n);
+  }
+
+  void sweep(double* residual_cell, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
+    HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
+    const dev::View<F> v = view();
+    for (int k0 = 0; k0 < nv_; k0 += chunk_) {
+      dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
+      const long long items = static_cast<long long>(nc_) * tiles_;
+      const unsigned grid = static_cast<unsigned>((items + 7) / 8 * 8);
+      begin(kGradient);
+      dev::k_gradient<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_);
+      HIPCHK(hipGetLastError());
+      end();
+      begin(kResidual);
+      dev::k_residual<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, rres_.p, partial_.p);
+      dev::k_residual_reduce<<<(nc_ + 255) / 256, 256, 0, stream_>>>(partial_.p, tiles_, nc_, residual_.p);
+      HIPCHK(hipGetLastError());
+      end();
+      const long long lanes = static_cast<long long>(n_blocks_) * ch.kc;
+      const unsigned sgrid = static_cast<unsigned>((lanes + 63) / 64);
+      F* cur = bufA_.p;
+      F* prev = nullptr;
+      for (int pass = 0; pass < block_passes_; ++pass) {
+        begin(kForward);
+        dev::k_sweep_forward<F><<<sgrid, 64, 0, stream_>>>(v, ch.k0, ch.kc, ch.stride, n_blocks_, block_off_.p, rres_.p, cur,
+                                                           prev);
+        HIPCHK(hipGetLastError());
+        end();
+        begin(kBackward);
+        dev::k_sweep_backward<F><<<sgrid, 64, 0, stream_>>>(v, ch.k0, ch.kc, ch.stride, n_blocks_, block_off_.p, cur,
+                                                            prev != nullptr ? 1 : 0, pass + 1 == block_passes_ ? 1 : 0);
+        HIPCHK(hipGetLastError());
+        end();
+        prev = cur;
+        cur = cur == bufA_.p ? bufB_.p : bufA_.p;
+      }
+    }
+    download(residual_cell, residual_.p, n);
+  }
+
+  long long gsis_correction(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
+                            std::size_t n_flags) override {
+    GSK_CHECK(n_w == static_cast<std::size_t>(nc_) * 5, "GSIS correction states length differs from 5 x cells");
+    GSK_CHECK(n_flags == 0 || n_flags == static_cast<std::size_t>(nc_), "flags length differs from the cell count");
+    begin(kOther);
+    DBuf<double> wh, wn;
+    wh.upload(w_half, n_w, stream_);
+    wn.upload(w_new, n_w, stream_);
+    dev::k_gsis_correction<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), wh.p, wn.p, negative_mode_, shift_.p, flags_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    std::vector<int> h(static_cast<std::size_t>(nc_));
+    download(h.data(), flags_.p, h.size());
+    long long replaced = 0;
+    for (std::size_t c = 0; c < h.size(); ++c) {
+      if (h[c] != 0) ++replaced;
+      if (n_flags) flags[c] = h[c];
+    }
+    return replaced;
+  }
+
+  void internal_face_mass_flux(const int* cells, const int* entries, std::size_t n, double* out) override {
+    if (n == 0) return;
+    std::vector<int> off(static_cast<std::size_t>(nc_) + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), csr_off_.p, off.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (std::size_t i = 0; i < n; ++i) {
+      GSK_CHECK(cells[i] >= 0 && cells[i] < nc_ && entries[i] >= off[static_cast<std::size_t>(cells[i])] &&
+                    entries[i] < off[static_cast<std::size_t>(cells[i]) + 1],
+                "internal face list refers to an entry outside its cell");
+    }
+    begin(kOther);
+    DBuf<int> dc, de;
+    DBuf<double> dout;
+    dc.upload(cells, n, stream_);
+    de.upload(entries, n, stream_);
+    dout.alloc(n);
+    dev::k_face_mass_flux<F><<<static_cast<unsigned>(std::min<std::size_t>(n, 65536)), dev::kBlock, 0, stream_>>>(
+        view(), dc.p, de.p, static_cast<int>(n), dout.p);
+    HIPCHK(hipGetLastError());
+    end();
+    download(out, dout.p, n);
+  }
+
+  gsk_timings take_timings() override {
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (auto& e : events_) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+      timings_[e.phase] += 1e-3 * ms;
+      free_events_.push_back(e.a);
+      free_events_.push_back(e.b);
+    }
+    events_.clear();
+    gsk_timings t{timings_[kMoments], timings_[kBoundary], timings_[kGradient], timings_[kResidual],
+                  timings_[kForward], timings_[kBackward], timings_[kOther]};
+    for (int i = 0; i < kNumPhases; ++i) timings_[i] = 0.0;
+    return t;
+  }
+
+ private:
+  struct EventPair {
+    hipEvent_t a, b;
+    int phase;
+  };
+
+  unsigned grid_cells() const { return static_cast<unsigned>(std::min(nc_, 1 << 20)); }
+  std::size_t pinned_bytes() const {
+    return std::max<std::size_t>({static_cast<std::size_t>(nc_) * dev::kMoments * sizeof(double),
+                                  (static_cast<std::size_t>(nb_) * dev::kBoundarySums + 1) * sizeof(double),
+                                  static_cast<std::size_t>(nc_) * sizeof(double), std::size_t(1) << 20});
+  }
+
+  template <class T>
+  void download(T* dst, const T* src, std::size_t n) {
+    const std::size_t bytes = n * sizeof(T);
+    if (bytes <= pinned_bytes()) {
+      HIPCHK(hipMemcpyAsync(pinned_, src, bytes, hipMemcpyDeviceToHost, stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+      std::memcpy(dst, pinned_, bytes);
+    } else {
+      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+
+  void convert_in(const double* f, std::size_t n) {
+    const std::size_t step = std::size_t(1) << 24;
+    DBuf<double> stage;
+    stage.alloc(std::min(n, step));
+    for (std::size_t i = 0; i < n; i += step) {
+      const std::size_t m = std::min(step, n - i);
+      HIPCHK(hipMemcpyAsync(stage.p, f + i, m * sizeof(double), hipMemcpyHostToDevice, stream_));
+      dev::k_from_double<F><<<1024, 256, 0, stream_>>>(stage.p, f_.p + i, static_cast<long long>(m));
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+  void convert_out(double* f, std::size_t n) {
+    const std::size_t step = std::size_t(1) << 24;
+    DBuf<double> stage;
+    stage.alloc(std::min(n, step));
+    for (std::size_t i = 0; i < n; i += step) {
+      const std::size_t m = std::min(step, n - i);
+      dev::k_to_double<F><<<1024, 256, 0, stream_>>>(f_.p + i, stage.p, static_cast<long long>(m));
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(f + i, stage.p, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+
+  void begin(int phase) {
+    EventPair e;
+    e.a = take_event();
+    e.b = take_event();
+    e.phase = phase;
+    HIPCHK(hipEventRecord(e.a, stream_));
+    events_.push_back(e);
+  }
+  void end() { HIPCHK(hipEventRecord(events_.back().b, stream_)); }
+  hipEvent_t take_event() {
+    if (!free_events_.empty()) {
+      hipEvent_t e = free_events_.back();
+      free_events_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+
+  dev::View<F> view() {
+    dev::View<F> v{};
+    v.n_cells = nc_;
+    v.nv = nv_;
+    v.n_bfaces = nb_;
+    v.xi_x = xi_x_.p;
+    v.xi_y = xi_y_.p;
+    v.xi_z = xi_z_.p;
+    v.weight = static_cast<F>(weight_);
+    v.weight_d = weight_;
+    v.csr_off = csr_off_.p;
+    v.csr = csr_.p;
+    v.eps2 = eps2_.p;
+    v.limiter_mode = limiter_mode_;
+    v.positivity = positivity_;
+    v.diag_scale = static_cast<F>(diag_scale_);
+    v.interface_damping = static_cast<F>(interface_damping_);
+    v.bf = bfaces_.p;
+    v.inflow = inflow_.p;
+    v.cells = cells_.p;
+    v.f = f_.p;
+    v.mw_off = n_mw_ > 0 ? mw_off_.p : nullptr;
+    v.mw_frac = n_mw_ > 0 ? mw_frac_.p : nullptr;
+    v.specular = specular_.p;
+    v.specular_first = specular_first_.p;
+    v.grid_to_storage = grid_to_storage_.p;
+    for (int a = 0; a < 3; ++a) v.grid_n[a] = grid_n_[a];
+    v.n_sweep_dirs = n_sweep_dirs_;
+    v.sweep_order = sweep_order_.p;
+    v.sweep_pos = sweep_pos_.p;
+    v.sweep_dir_of = sweep_dir_of_.p;
+    v.deq = deq_;
+    v.discrete_eq = discrete_eq_;
+    v.deq_fallback = deq_fallback_;
+    v.cold_bgk = cold_bgk_;
+    return v;
+  }
+
+  hipStream_t stream_ = nullptr;
+  int nc_ = 0, nv_ = 0, nb_ = 0, n_blocks_ = 1, block_passes_ = 1, negative_mode_ = 0;
+  int chunk_ = 0, tiles_ = 0, n_sweep_dirs_ = 0, n_mw_ = 0;
+  int grid_n_[3] = {0, 0, 0};
+  double mu0_ = 0, omega_ = 0, Pr_ = 0, cold_bgk_ = 0, diag_scale_ = 1, interface_damping_ = 0, weight_ = 0;
+  int discrete_eq_ = 0, deq_fallback_ = 0, limiter_mode_ = 1, positivity_ = 0;
+  dev::DeqGrid deq_{};
+  DBuf<F> xi_x_, xi_y_, xi_z_;
+  DBuf<double> xd_, yd_, zd_;
+  DBuf<int> csr_off_;
+  DBuf<dev::CsrE<F>> csr_;
+  DBuf<F> eps2_;
+  DBuf<double> volume_, vdt_;
+  DBuf<dev::BFace<F>> bfaces_;
+  DBuf<dev::Inflow<F>> inflow_;
+  DBuf<int> block_off_, sweep_order_, sweep_pos_;
+  DBuf<unsigned char> sweep_dir_of_;
+  DBuf<long long> mw_off_;
+  DBuf<F> mw_frac_, specular_, specular_first_;
+  DBuf<int> grid_to_storage_;
+  DBuf<F> f_;
+  DBuf<dev::CellP<F>> cells_;
+  DBuf<double> moments_, shift_, residual_, bsums_, partial_;
+  DBuf<int> flags_;
+  DBuf<F> gx_, gy_, gz_, rres_, bufA_, bufB_;
+  std::vector<int> deq_flags_;
+  std::vector<int> host_bface_kind_;
+  void* pinned_ = nullptr;
+  std::vector<EventPair> events_;
+  std::vector<hipEvent_t> free_events_;
+  double timings_[kNumPhases];
+};
+
+}  // namespace
+
+std::unique_ptr<Backend> make_hip_backend(const gsk_setup_view& view) {
+  int ndev = 0;
+  const hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    throw Error("no HIP device is available; the kinetic backend runs on the GPU only");
+  }
+  if (view.precision_fp64) return std::make_unique<HipBackend<double>>(view);
+  return std::make_unique<HipBackend<float>>(view);
+}
+
+}  // namespace gsk

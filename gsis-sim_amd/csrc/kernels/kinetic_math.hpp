@@ -1,0 +1,389 @@
+#pragma once
+// Per (cell, velocity) arithmetic of the kinetic step for the HIP kernels, templated on the
+// storage precision F of f (double for the drop-in FP64 form, float for the mixed form).
+//
+// Each function restates one reference formula; the operation order is kept so that the FP64
+// form agrees with the reference CPU path to rounding (all files relative to the reference tree):
+//   Maxwellian / Shakhov params         src/physics/maxwell.hpp:17-64
+//   Shakhov target with conservation    src/kinetic/kinetic_types.hpp:68-90
+//   Venkatakrishnan limiter             src/physics/limiter.hpp:17-50
+//   LSQ gradient + limiter              src/kinetic/kinetic_kernels.hpp:145-201
+//   residual (2nd-order upwind)         src/kinetic/kinetic_kernels.hpp:238-278
+//   LU-SGS forward/backward cell        src/kinetic/kinetic_kernels.hpp:290-358
+//   moment terms / finalize             src/kinetic/kinetic_kernels.hpp:378-442
+//   cell params                         src/kinetic/kinetic_kernels.hpp:451-471
+//   discrete equilibrium Newton         src/kinetic/discrete_equilibrium.hpp:25-188,
+//                                       src/kinetic/kinetic_kernels.hpp:489-493
+//   GSIS-correction helpers             src/kinetic/kinetic_kernels.hpp:557-587
+//   boundary terms / reflections        src/kinetic/kinetic_kernels.hpp:76-133, :596-609
+
+#include <cmath>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GSK_HD __host__ __device__ __forceinline__
+#else
+#define GSK_HD inline
+#endif
+
+namespace gsk::dev {
+
+inline constexpr double kPi = 3.14159265358979323846;
+inline constexpr int kMomentSums = 13;
+inline constexpr int kMoments = 15;
+inline constexpr int kBoundarySums = 20;
+inline constexpr int kDeqSums = 20;
+
+enum : int { kBcDiffuse = 0, kBcPressure = 1, kBcVacuum = 2, kBcSymmetry = 3, kBcPeriodic = 4, kBcMaxwell = 5 };
+enum : int { kLimNone = 0, kLimAbs = 1, kLimRel = 2 };
+
+GSK_HD double dexp(double x) { return ::exp(x); }
+GSK_HD float dexp(float x) { return ::expf(x); }
+GSK_HD double dsqrt(double x) { return ::sqrt(x); }
+GSK_HD float dsqrt(float x) { return ::sqrtf(x); }
+GSK_HD double dabs(double x) { return ::fabs(x); }
+GSK_HD float dabs(float x) { return ::fabsf(x); }
+template <class T>
+GSK_HD T dmin(T a, T b) { return b < a ? b : a; }
+template <class T>
+GSK_HD T dmax(T a, T b) { return a < b ? b : a; }
+
+// f_eq = A exp(-B |xi - u|^2)
+template <class F>
+struct Maxw {
+  F ux, uy, uz, A, B;
+};
+
+template <class F>
+GSK_HD F maxw_eval(const Maxw<F>& m, F x, F y, F z) {
+  const F cx = x - m.ux, cy = y - m.uy, cz = z - m.uz;
+  return m.A * dexp(-m.B * (cx * cx + cy * cy + cz * cz));
+}
+
+// (rho, u, T) -> (A, B) with R = 1, computed in double and stored in F
+template <class F>
+GSK_HD Maxw<F> maxw_from_state(double rho, double ux, double uy, double uz, double T) {
+  const double theta = T;
+  const double tpt = (2.0 * kPi) * theta;
+  Maxw<F> m;
+  m.ux = static_cast<F>(ux);
+  m.uy = static_cast<F>(uy);
+  m.uz = static_cast<F>(uz);
+  m.A = static_cast<F>(rho / (tpt * ::sqrt(tpt)));
+  m.B = static_cast<F>(1.0 / (2.0 * theta));
+  return m;
+}
+
+// Shakhov target f_S = eq [1 + coef (c.q)(|c|^2/theta - 5) + g0 + g.c + g4 |c|^2/2], c = xi - uc
+template <class F>
+struct Target {
+  Maxw<F> eq;
+  F ucx, ucy, ucz;
+  F qx, qy, qz;
+  F theta, coef;
+  F g0, g4, gx, gy, gz;
+};
+
+template <class F>
+struct CellP {
+  Target<F> s;
+  F coll;  // V / tau
+  F diag;  // V / dt + V / tau
+};
+
+template <class F>
+GSK_HD F shakhov_poly(const Target<F>& s, F cx, F cy, F cz, F c2) {
+  return s.coef * (cx * s.qx + cy * s.qy + cz * s.qz) * (c2 / s.theta - F(5));
+}
+
+template <class F>
+GSK_HD F target_eval(const Target<F>& s, F x, F y, F z) {
+  const F cx = x - s.ucx, cy = y - s.ucy, cz = z - s.ucz;
+  const F c2 = cx * cx + cy * cy + cz * cz;
+  const F m = maxw_eval(s.eq, x, y, z);
+  return m * (F(1) + shakhov_poly(s, cx, cy, cz, c2) + (s.g0 + (cx * s.gx + cy * s.gy + cz * s.gz) + F(0.5) * s.g4 * c2));
+}
+
+// cell params from 15 moments (kinetic_kernels.hpp:451-471)
+template <class F>
+GSK_HD CellP<F> make_cell_params(const double* m, double volume, double vdt, double mu0, double omega, double Pr,
+                                 double cold_bgk_T) {
+  const Maxw<double> eqd = maxw_from_state<double>(m[0], m[1], m[2], m[3], m[4]);
+  const double theta = m[4];
+  const double p = m[0] * theta;
+  const double coef = (1.0 - Pr) / (5.0 * p * theta);
+  CellP<F> cp;
+  cp.s.eq.ux = static_cast<F>(eqd.ux);
+  cp.s.eq.uy = static_cast<F>(eqd.uy);
+  cp.s.eq.uz = static_cast<F>(eqd.uz);
+  cp.s.eq.A = static_cast<F>(eqd.A);
+  cp.s.eq.B = static_cast<F>(eqd.B);
+  cp.s.ucx = cp.s.eq.ux;
+  cp.s.ucy = cp.s.eq.uy;
+  cp.s.ucz = cp.s.eq.uz;
+  cp.s.qx = static_cast<F>(m[12]);
+  cp.s.qy = static_cast<F>(m[13]);
+  cp.s.qz = static_cast<F>(m[14]);
+  cp.s.theta = static_cast<F>(theta);
+  cp.s.coef = m[4] < cold_bgk_T ? F(0) : static_cast<F>(coef);
+  cp.s.g0 = F(0);
+  cp.s.g4 = F(0);
+  cp.s.gx = cp.s.gy = cp.s.gz = F(0);
+  const double mu = mu0 * ::pow(m[4], omega);
+  const double coll = volume * m[5] / mu;
+  cp.coll = static_cast<F>(coll);
+  cp.diag = static_cast<F>(coll + vdt);
+  return cp;
+}
+
+// ---- Venkatakrishnan limiter ----
+template <class F>
+GSK_HD F venkat_face(F d2, F dmaxv, F dminv, F eps2) {
+  F d1;
+  if (d2 > F(0)) {
+    d1 = dmaxv;
+  } else if (d2 < F(0)) {
+    d1 = dminv;
+  } else {
+    return F(1);
+  }
+  const F num = (d1 * d1 + eps2) + F(2) * d1 * d2;
+  const F den = d1 * d1 + F(2) * d2 * d2 + d1 * d2 + eps2;
+  if (!(den > F(0))) return F(0);
+  const F r = num / den;
+  return r < F(0) ? F(0) : (F(1) < r ? F(1) : r);
+}
+
+// ---- moments about the per-cell shift u0 ----
+template <class F>
+GSK_HD void moment_terms(F f, F cx, F cy, F cz, F (&t)[kMomentSums]) {
+  t[0] = f;
+  t[1] = f * cx;
+  t[2] = f * cy;
+  t[3] = f * cz;
+  t[4] = t[1] * cx;
+  t[5] = t[2] * cy;
+  t[6] = t[3] * cz;
+  t[7] = t[1] * cy;
+  t[8] = t[2] * cz;
+  t[9] = t[1] * cz;
+  const F fc2 = f * (cx * cx + cy * cy + cz * cz);
+  t[10] = fc2 * cx;
+  t[11] = fc2 * cy;
+  t[12] = fc2 * cz;
+}
+
+// 13 raw sums (before the weight) and the shift -> 15 moments; rho <= 0 leaves out[1..] = 0
+GSK_HD void finalize_moments(const double* s, double w, double u0x, double u0y, double u0z, double* out) {
+  const double s0 = s[0];
+  if (!(s0 > 0.0)) {
+    out[0] = w * s0;
+    for (int k = 1; k < kMoments; ++k) out[k] = 0.0;
+    return;
+  }
+  const double dx = s[1] / s0, dy = s[2] / s0, dz = s[3] / s0;
+  const double rho = w * s0;
+  const double pxx = w * (s[4] - s0 * dx * dx);
+  const double pyy = w * (s[5] - s0 * dy * dy);
+  const double pzz = w * (s[6] - s0 * dz * dz);
+  const double pxy = w * (s[7] - s0 * dx * dy);
+  const double pyz = w * (s[8] - s0 * dy * dz);
+  const double pxz = w * (s[9] - s0 * dx * dz);
+  const double p = (pxx + pyy + pzz) / 3.0;
+  const double tr2 = s[4] + s[5] + s[6];
+  const double du2 = dx * dx + dy * dy + dz * dz;
+  const double s2dx = s[4] * dx + s[7] * dy + s[9] * dz;
+  const double s2dy = s[7] * dx + s[5] * dy + s[8] * dz;
+  const double s2dz = s[9] * dx + s[8] * dy + s[6] * dz;
+  out[0] = rho;
+  out[1] = u0x + dx;
+  out[2] = u0y + dy;
+  out[3] = u0z + dz;
+  out[4] = p / rho;
+  out[5] = p;
+  out[6] = pxx - p;
+  out[7] = pyy - p;
+  out[8] = pzz - p;
+  out[9] = pxy;
+  out[10] = pyz;
+  out[11] = pxz;
+  out[12] = 0.5 * w * (s[10] - dx * tr2 - 2.0 * s2dx + 2.0 * s0 * du2 * dx);
+  out[13] = 0.5 * w * (s[11] - dy * tr2 - 2.0 * s2dy + 2.0 * s0 * du2 * dy);
+  out[14] = 0.5 * w * (s[12] - dz * tr2 - 2.0 * s2dz + 2.0 * s0 * du2 * dz);
+}
+
+// ---- discrete (conservative) equilibrium ----
+struct DeqGrid {
+  double lo[3], hi[3], d[3];
+  double criterion;
+  int max_iterations;
+  double tolerance;
+};
+
+GSK_HD bool deq_resolved(const DeqGrid& g, double ux, double uy, double uz, double T) {
+  if (!(T > 0.0)) return false;
+  const double u[3] = {ux, uy, uz};
+  for (int a = 0; a < 3; ++a) {
+    if (2.0 * kPi * kPi * T / (g.d[a] * g.d[a]) < g.criterion) return false;
+    const double m1 = g.hi[a] - u[a];
+    const double m2 = u[a] - g.lo[a];
+    if (!(m1 > 0.0 && m2 > 0.0)) return false;
+    if (m1 * m1 < 2.0 * T * g.criterion || m2 * m2 < 2.0 * T * g.criterion) return false;
+  }
+  return true;
+}
+
+template <class F>
+GSK_HD void deq_terms(F e, F pm, F cx, F cy, F cz, F (&t)[kDeqSums]) {
+  const F h = F(0.5) * (cx * cx + cy * cy + cz * cz);
+  t[0] = e;
+  t[1] = e * cx;
+  t[2] = e * cy;
+  t[3] = e * cz;
+  t[4] = e * h;
+  t[5] = t[1] * cx;
+  t[6] = t[1] * cy;
+  t[7] = t[1] * cz;
+  t[8] = t[1] * h;
+  t[9] = t[2] * cy;
+  t[10] = t[2] * cz;
+  t[11] = t[2] * h;
+  t[12] = t[3] * cz;
+  t[13] = t[3] * h;
+  t[14] = t[4] * h;
+  const F ep = e * pm;
+  t[15] = ep;
+  t[16] = ep * cx;
+  t[17] = ep * cy;
+  t[18] = ep * cz;
+  t[19] = ep * h;
+}
+
+GSK_HD void deq_jacobian(const double* s, double w, double (&J)[5][5]) {
+  int k = 0;
+  for (int a = 0; a < 5; ++a)
+    for (int b = a; b < 5; ++b) {
+      J[a][b] = w * s[k];
+      J[b][a] = J[a][b];
+      ++k;
+    }
+}
+
+// partial-pivot Gauss elimination; false when singular or NaN
+GSK_HD bool deq_solve5(double (&A)[5][5], double (&b)[5], double (&x)[5]) {
+  for (int k = 0; k < 5; ++k) {
+    int piv = k;
+    double best = ::fabs(A[k][k]);
+    for (int r = k + 1; r < 5; ++r)
+      if (::fabs(A[r][k]) > best) {
+        best = ::fabs(A[r][k]);
+        piv = r;
+      }
+    if (!(best > 0.0)) return false;
+    if (piv != k) {
+      for (int c = 0; c < 5; ++c) {
+        const double tmp = A[k][c];
+        A[k][c] = A[piv][c];
+        A[piv][c] = tmp;
+      }
+      const double tb = b[k];
+      b[k] = b[piv];
+      b[piv] = tb;
+    }
+    for (int r = k + 1; r < 5; ++r) {
+      const double f = A[r][k] / A[k][k];
+      for (int c = k; c < 5; ++c) A[r][c] -= f * A[k][c];
+      b[r] -= f * b[k];
+    }
+  }
+  for (int k = 4; k >= 0; --k) {
+    double sum = b[k];
+    for (int c = k + 1; c < 5; ++c) sum -= A[k][c] * x[c];
+    x[k] = sum / A[k][k];
+    if (!(x[k] == x[k])) return false;
+  }
+  return true;
+}
+
+GSK_HD void deq_initial_beta(double rho, double T, double (&beta)[5]) {
+  const double tpt = 2.0 * kPi * T;
+  beta[0] = ::log(rho / (tpt * ::sqrt(tpt)));
+  beta[1] = 0.0;
+  beta[2] = 0.0;
+  beta[3] = 0.0;
+  beta[4] = -1.0 / T;
+}
+
+template <class F>
+GSK_HD Maxw<F> deq_params(const double (&beta)[5], double ux, double uy, double uz) {
+  const double B = -0.5 * beta[4];
+  const double dx = -beta[1] / beta[4];
+  const double dy = -beta[2] / beta[4];
+  const double dz = -beta[3] / beta[4];
+  Maxw<F> m;
+  m.ux = static_cast<F>(ux + dx);
+  m.uy = static_cast<F>(uy + dy);
+  m.uz = static_cast<F>(uz + dz);
+  m.A = static_cast<F>(::exp(beta[0] + B * (dx * dx + dy * dy + dz * dz)));
+  m.B = static_cast<F>(B);
+  return m;
+}
+
+// 1 = converged (beta unchanged), 0 = beta updated, -1 = failure; rel = relative moment error
+GSK_HD int deq_newton_step(const double* s, double w, double rho, double T, double tol, double (&beta)[5],
+                           double& rel) {
+  double G[5] = {w * s[0] - rho, w * s[1], w * s[2], w * s[3], w * s[4] - 1.5 * rho * T};
+  const double su = rho * ::sqrt(T);
+  rel = dmax(::fabs(G[0]) / rho,
+             dmax(dmax(::fabs(G[1]), dmax(::fabs(G[2]), ::fabs(G[3]))) / su, ::fabs(G[4]) / (rho * T)));
+  if (!(rel == rel)) return -1;
+  if (rel < tol) return 1;
+  double J[5][5];
+  deq_jacobian(s, w, J);
+  double rhs[5] = {-G[0], -G[1], -G[2], -G[3], -G[4]};
+  double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (!deq_solve5(J, rhs, d)) return -1;
+  double scale = 1.0;
+  if (::fabs(d[4]) > 0.5 * ::fabs(beta[4])) scale = 0.5 * ::fabs(beta[4]) / ::fabs(d[4]);
+  if (::fabs(d[0]) * scale > 1.0) scale = 1.0 / ::fabs(d[0]);
+  for (int k = 0; k < 5; ++k) beta[k] += scale * d[k];
+  if (!(beta[4] < 0.0)) return -1;
+  return 0;
+}
+
+GSK_HD bool deq_shakhov_correction(const double* s, double w, double (&gamma)[5]) {
+  double J[5][5];
+  deq_jacobian(s, w, J);
+  double rhs[5] = {-w * s[15], -w * s[16], -w * s[17], -w * s[18], -w * s[19]};
+  return deq_solve5(J, rhs, gamma);
+}
+
+GSK_HD bool deq_accept(int r, double rel, double rel_prev, bool last) {
+  if (r == 1) return true;
+  if (r == 0 && rel < 1e-4 && (rel > 0.5 * rel_prev || last)) return true;
+  return false;
+}
+
+// ---- GSIS correction helpers ----
+template <class F>
+GSK_HD F negative_check_value(F old, F x, bool created) {
+  return created ? x - dmin(old, F(0)) : x;
+}
+template <class F>
+GSK_HD F correction_alpha_limit(F old, F d, F thr) {
+  if (!(d < F(0))) return F(1);
+  const F room = old - (dmin(old, F(0)) + thr);
+  return dmin(F(1), room / (-d));
+}
+template <class F>
+GSK_HD F blend_theta(F old, F d, F e, F thr) {
+  const F g = old + d;
+  const F bound = dmin(old, F(0)) + thr;
+  if (!(g < bound)) return F(0);
+  const F den = e - g;
+  if (!(den > F(0))) return F(1);
+  return dmin(F(1), (bound - g) / den);
+}
+
+}  // namespace gsk::dev
