@@ -1,0 +1,184 @@
+"""Throughput of the kinetic step on one MI355X: cell-velocity updates per second.
+
+Workload (default): the 3D tube-to-vacuum case -- O-grid hex cylinder R = L_ref, length 5R,
+~40k cells (16x16 core + 4x16x10 ring, 45 axial), pressure inlet p_in / T0, vacuum outlet, diffuse
+wall, delta_rp = 10, 24^3 uniform velocity grid on [-6, 6], FP64 (the plain drop-in form), kinetic
+defaults (octant sweeps, 4 inner sweeps with wall update, 2 block passes, discrete equilibrium).
+Synthetic mesh from this package's generator; f starts as the rest equilibrium at (p_in, T0).
+
+A step is one CIS iteration (prepare + sweep). One update = one (cell, velocity) pair through one
+inner LU-SGS sweep (gradient + residual + block passes + f update), so a step does
+cells x velocities x inner_sweeps updates. Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line; per-kernel HIP-event timings of the backend's own stream give the roofline of
+the dominant kernel; the CPU oracle is timed on the host on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gsis-sim_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def _package():
+    spec = importlib.util.spec_from_file_location("gsis_sim_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gsis_sim_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+WORKLOADS = {
+    # name: (tube args, velocity points per axis, precision)
+    "tube40k_24cubed_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, "fp64"),
+    "tube40k_24cubed_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, "fp32"),
+    "tube135k_32cubed_fp32": (dict(n_core=24, n_ring=14, n_axial=70), 32, "fp32"),
+}
+
+# algorithmic bytes per (cell, velocity) pair and launch, in units of sizeof(f)
+#   gradient: read f, write 3 gradient components                      -> 4
+#   residual: read f and 3 gradient components, write R                -> 5
+#   forward:  read R, write dF* (pass 0); + read previous dF (pass > 0)  -> 2 / 3
+#   backward: read dF*, write dF; + read and write f (last pass)        -> 2 / 4
+def phase_bytes_per_pair(phase, passes):
+    if phase == "gradient":
+        return 4.0
+    if phase == "residual":
+        return 5.0
+    if phase == "sweep_forward":
+        return (2.0 + 3.0 * (passes - 1)) / passes
+    if phase == "sweep_backward":
+        return (2.0 * (passes - 1) + 4.0) / passes
+    return None
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, case_dir, tube_args, precision, iterations=3):
+    """Oracle (CPU restatement, OpenMP) on the same tube mesh with a 6^3 velocity subset."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import gsis_sim_amd.meshgen as mg
+
+    threads = oracle.set_threads(min(16, os.cpu_count() or 1))
+    path = mg.tube_case(case_dir, n_vel=6, vrange=6.0, delta=10.0, kinetic={"precision": precision}, name="cpu_tube",
+                        **tube_args)
+    s = g.Setup(path)
+    cb = oracle.CpuBackend(s, "oracle", ops_type=g.BackendOps)
+    sv = g.Solver(s, cb.ops)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    sv.prepare()
+    sv.sweep()  # first iteration excluded, as the reference's timing does
+    t0 = time.perf_counter()
+    for _ in range(iterations):
+        sv.prepare()
+        sv.sweep()
+    dt = time.perf_counter() - t0
+    inner = 4
+    updates = s.n_cells * s.nv * inner * iterations
+    sv.close()
+    return dict(value=updates / dt / 1e9, unit="Gcell·vel updates/s", cores=threads, kind="port",
+                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a 6^3 velocity grid, "
+                       f"{iterations} CIS iterations after 1 untimed ({dt:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="tube40k_24cubed_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the kinetic step is built for one MI355X in this version "
+              "(multi-GPU velocity sharding is a design, not implemented); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    g = _package()
+    import gsis_sim_amd.meshgen as mg
+
+    tube_args, nvel, precision = WORKLOADS[args.workload]
+    s_bytes = 8 if precision == "fp64" else 4
+    with tempfile.TemporaryDirectory() as d:
+        log(f"generating {args.workload}")
+        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=10.0, kinetic={"precision": precision}, **tube_args)
+        setup = g.Setup(path)
+        log(f"setup: {setup.n_cells} cells, {setup.nv} velocities, {setup.n_blocks} blocks")
+        sv = g.Solver(setup)
+        be = sv.backend()
+        sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+        passes = setup.view().block_passes
+        inner = 4
+        for i in range(args.warmup):
+            sv.prepare()
+            info = sv.sweep()
+            log(f"warmup {i + 1}/{args.warmup}: residual {info['residual']:.4e}")
+        torch.cuda.synchronize()
+        phase = {}
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            sv.prepare()
+            info = sv.sweep()
+            for k, v in info["timings"].items():
+                phase[k] = phase.get(k, 0.0) + v
+            log(f"step {i + 1}/{args.steps}: residual {info['residual']:.4e}")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        chunk = be.chunk_width()
+        n_chunks = -(-setup.nv // chunk)
+        sv.close()
+
+        pairs = setup.n_cells * setup.nv
+        updates = pairs * inner * args.steps
+        value = updates / elapsed / 1e9
+        # launches per step of each phase (HIP events bracket each launch group on the backend stream)
+        launches = {"gradient": inner * n_chunks, "residual": inner * n_chunks,
+                    "sweep_forward": inner * n_chunks * passes, "sweep_backward": inner * n_chunks * passes}
+        kernel_phases = {k: phase.get(k, 0.0) for k in launches}
+        dom = max(kernel_phases, key=kernel_phases.get)
+        n_launch = launches[dom] * args.steps
+        avg_launch_s = kernel_phases[dom] / n_launch
+        bytes_per_launch = phase_bytes_per_pair(dom, passes) * s_bytes * pairs / n_chunks
+        achieved = bytes_per_launch / avg_launch_s / 1e9
+        b_alg = (2 + 2 * passes) * s_bytes
+        result = {
+            "metric": "Gcell·vel updates/s", "value": round(value, 4), "unit": "Gcell·vel updates/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if s_bytes == 8 else "f32",
+            "data": "synthetic",
+            "config": {"workload": args.workload, "cells": setup.n_cells, "velocities": setup.nv,
+                       "inner_sweeps": inner, "block_passes": passes, "sweep_blocks": setup.n_blocks,
+                       "chunk_width": chunk, "delta_rp": 10.0},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "avg_launch_ms": round(1e3 * avg_launch_s, 4),
+                         "bytes_per_launch": int(bytes_per_launch)},
+            "step_roofline": {"bytes_per_update": b_alg, "achieved": round(value * b_alg, 2),
+                              "frac": round(value * b_alg / HBM_PEAK_GBS, 4)},
+            "phase_seconds_per_step": {k: round(v / args.steps, 5) for k, v in phase.items()},
+        }
+        if not args.no_cpu_baseline:
+            log("cpu baseline")
+            result["cpu_baseline"] = cpu_baseline(g, d, tube_args, precision)
+        else:
+            result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -20,15 +20,6 @@ import struct
 
 import numpy as np
 
-# hex corner order: bit 0 -> i, bit 1 -> j, bit 2 -> k
-_HEX_FACES = np.array([[0, 2, 6, 4], [1, 3, 7, 5], [0, 1, 5, 4], [2, 3, 7, 6], [0, 1, 3, 2], [4, 5, 7, 6]])
-
-
-def _cyclic(face_corners):
-    # reorder a quad given as (a, b, c, d) with a-b and c-d opposite edges to a cycle
-    return face_corners
-
-
 def hex_to_poly(points: np.ndarray, cells: np.ndarray, patch_of_face, patch_names, patch_kinds):
     """Convert hexahedra (8 corners, corner index bits = (i, j, k)) to the .pmesh face layout.
 
@@ -178,6 +169,41 @@ def slab_mesh(nx: int = 200, length: float = 1.0e-3, thickness: float | None = N
 
     return hex_to_poly(pts, cells, patch_of, ["wall_left", "wall_right", "sym_y", "sym_z"],
                        ["wall", "wall", "symmetry", "symmetry"])
+
+
+def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: float = 0.0,
+             seed: int = 0) -> dict:
+    """Structured box of hexahedra, optionally with interior nodes moved by up to `distort` x the
+    local spacing (seeded numpy generator; boundary nodes stay on their planes so every boundary
+    face keeps an axis-aligned normal). Patches xmin, xmax, ymin, ymax, zmin, zmax."""
+    n = (nx, ny, nz)
+    axes = [np.linspace(0.0, size[a], n[a] + 1) for a in range(3)]
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    idx = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    pts = np.stack([axes[a][idx[:, a]] for a in range(3)], 1)
+    if distort > 0.0:
+        rng = np.random.default_rng(seed)
+        for a in range(3):
+            h = size[a] / n[a]
+            interior = (idx[:, a] > 0) & (idx[:, a] < n[a])
+            pts[interior, a] += distort * h * rng.uniform(-1.0, 1.0, interior.sum())
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k  # noqa: E731
+    cells = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                cells.append([nid(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) for c in range(8)])
+
+    def patch_of(fc, nrm):
+        p = np.full(len(fc), -1)
+        for a in range(3):
+            p[fc[:, a] < 1e-9 * size[a]] = 2 * a
+            p[fc[:, a] > size[a] * (1 - 1e-9)] = 2 * a + 1
+        assert (p >= 0).all()
+        return p
+
+    names = ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
+    return hex_to_poly(pts, np.array(cells), patch_of, names, ["patch"] * 6)
 
 
 def tube_mesh(n_core: int = 16, n_ring: int = 10, n_axial: int = 45, radius: float = 1.0e-3,

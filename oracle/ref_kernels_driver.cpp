@@ -154,6 +154,7 @@ class RefBackend {
 
   void set_equilibrium(const double* st) {
     const KineticView<F> v = view();
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc_; ++c) {
       const double* s = st + static_cast<std::size_t>(c) * 5;
       const MaxwellParams<F> m = equilibrium_params(s[0], s[1], s[2], s[3], s[4]);
@@ -170,6 +171,7 @@ class RefBackend {
     for (std::size_t i = 0; i < f_.size(); ++i) f[i] = static_cast<double>(f_[i]);
   }
   void compute_moments(double* out) {
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc_; ++c) {
       const double u0x = shift_[c * 3], u0y = shift_[c * 3 + 1], u0z = shift_[c * 3 + 2];
       double sums[kMomentSums] = {};
@@ -199,6 +201,7 @@ class RefBackend {
   }
   void update_cell_params() {
     flags_.assign(nc_, 0);
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc_; ++c) {
       const double* m = &moments_[static_cast<std::size_t>(c) * kMomentsPerCell];
       CellParams<F> cp = make_cell_params<F>(m, sv_.volume[c], sv_.vdt[c], sv_.mu0, sv_.omega, sv_.Pr,
@@ -260,6 +263,7 @@ class RefBackend {
         }
       }
     }
+#pragma omp parallel for schedule(dynamic, 4)
     for (int b = 0; b < nb_; ++b) {
       const BoundaryFaceInfo<F>& bf = bfaces_[b];
       Accum<double> acc[kBoundarySumsPerFace];
@@ -307,6 +311,7 @@ class RefBackend {
       ch.gy = gy_.data();
       ch.gz = gz_.data();
       ch.r = r_.data();
+#pragma omp parallel for schedule(static)
       for (int c = 0; c < nc_; ++c)
         for (int t = 0; t < ch.kc; ++t) {
           const int p = k0 + t;
@@ -316,6 +321,7 @@ class RefBackend {
           ch.gy[wi] = g.y;
           ch.gz[wi] = g.z;
         }
+#pragma omp parallel for schedule(static)
       for (int c = 0; c < nc_; ++c) {
         double r2 = 0.0;
         for (int t = 0; t < ch.kc; ++t) {
@@ -335,6 +341,7 @@ class RefBackend {
           cp.dprev = gy_.data();
         }
         cp.update_f = pass + 1 == passes ? 1 : 0;
+#pragma omp parallel for schedule(static)
         for (int lane = 0; lane < n_blocks * ch.kc; ++lane) {
           const int b = lane / ch.kc, t = lane % ch.kc;
           const Vec3T<F> xi = velocity_at(v, k0 + t);
@@ -344,6 +351,7 @@ class RefBackend {
           sweep_order_of(v, k0 + t, order, pos);
           for (int k = c0; k < c1; ++k) sweep_forward_cell(v, cp, order ? order[k] : k, t, c0, c1, xi, pos);
         }
+#pragma omp parallel for schedule(static)
         for (int lane = 0; lane < n_blocks * ch.kc; ++lane) {
           const int b = lane / ch.kc, t = lane % ch.kc;
           const Vec3T<F> xi = velocity_at(v, k0 + t);

@@ -1,0 +1,120 @@
+"""Shared fixtures: the package (loaded from gsis-sim_amd/), the checker backends (oracle/), and
+small programmatic cases in the style of the reference's make_kinetic_case / make_box_mesh helpers
+(tests/kinetic/kinetic_test_cases.cpp:9-60, tests/support/box_mesh.cpp of the reference)."""
+
+from __future__ import annotations
+
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(ROOT, "gsis-sim_amd")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def load_package():
+    if "gsis_sim_amd" in sys.modules:
+        return sys.modules["gsis_sim_amd"]
+    spec = importlib.util.spec_from_file_location("gsis_sim_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gsis_sim_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")
+    config.addinivalue_line("markers", "slow: long CPU test")
+
+
+@pytest.fixture(scope="session")
+def gsk():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def meshgen(gsk):
+    import gsis_sim_amd.meshgen as mg  # noqa: F401  (registered by load_package)
+    return sys.modules["gsis_sim_amd.meshgen"]
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import oracle
+    if not os.path.exists(oracle.ORACLE_LIB):
+        oracle.build()
+    return oracle
+
+
+T0 = 273.15
+
+
+def box_case(mg, directory, n=(5, 3, 2), distort=0.25, seed=7, n_vel=8, vrange=5.0, kinetic=None, bcs="mixed",
+             delta=1.0, name="box", wall_T=(0.95, 1.05)):
+    """Distorted box with every boundary kind the kinetic path handles:
+    xmin pressure inlet, xmax pressure outlet (p > 0), ymin moving diffuse wall, ymax Maxwell wall
+    (alpha = 0.8, axis aligned -> mirror shortcut), zmin / zmax symmetry."""
+    mesh = mg.box_mesh(*n, distort=distort, seed=seed)
+    mg.write_pmesh(os.path.join(directory, f"{name}.pmesh"), mesh)
+    if bcs == "mixed":
+        bnd = [dict(patch="xmin", type="pressure_inlet", temperature=1.1 * T0),
+               dict(patch="xmax", type="pressure_outlet", pressure=1.0),  # Pa, tiny vs p_ref -> nearly vacuum-like
+               dict(patch="ymin", type="diffuse_wall", temperature=0.9 * T0, velocity=[20.0, 0.0, 0.0]),
+               dict(patch="ymax", type="maxwell_wall", accommodation=0.8, temperature=1.05 * T0),
+               dict(patch="zmin", type="symmetry"), dict(patch="zmax", type="symmetry")]
+    elif bcs == "closed":
+        bnd = [dict(patch="xmin", type="diffuse_wall", temperature=wall_T[0] * T0),
+               dict(patch="xmax", type="diffuse_wall", temperature=wall_T[1] * T0),
+               dict(patch="ymin", type="diffuse_wall", temperature=T0), dict(patch="ymax", type="diffuse_wall", temperature=T0),
+               dict(patch="zmin", type="symmetry"), dict(patch="zmax", type="symmetry")]
+    elif bcs == "vacuum":
+        bnd = [dict(patch="xmin", type="pressure_inlet", temperature=T0), dict(patch="xmax", type="vacuum"),
+               dict(patch="ymin", type="diffuse_wall", temperature=T0), dict(patch="ymax", type="diffuse_wall", temperature=T0),
+               dict(patch="zmin", type="symmetry"), dict(patch="zmax", type="symmetry")]
+    else:
+        raise ValueError(bcs)
+    path = os.path.join(directory, f"{name}.toml")
+    mg.write_case(path, f"{name}.pmesh", name, bnd, [n_vel] * 3, [(-vrange, vrange)] * 3,
+                  dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), kinetic)
+    return path
+
+
+def perturbed_distribution(setup, seed=1, amp=0.05):
+    """Local Maxwellians with varying (rho, u, T) times (1 + amp * noise): an f with nonzero
+    gradients, heat flux and stress, in the solver's internal numbering and storage order."""
+    rng = np.random.default_rng(seed)
+    v = setup.view()
+    nv = v.nv
+    xi = np.stack([np.ctypeslib.as_array(v.xi_x, (nv,)), np.ctypeslib.as_array(v.xi_y, (nv,)),
+                   np.ctypeslib.as_array(v.xi_z, (nv,))], 1)
+    nc = setup.n_cells
+    rho = rng.uniform(0.7, 1.3, nc)
+    T = rng.uniform(0.7, 1.3, nc)
+    u = rng.uniform(-0.3, 0.3, (nc, 3))
+    c2 = ((xi[None, :, :] - u[:, None, :]) ** 2).sum(-1)
+    f = rho[:, None] / (2 * np.pi * T[:, None]) ** 1.5 * np.exp(-c2 / (2 * T[:, None]))
+    f *= 1.0 + amp * rng.uniform(-1.0, 1.0, f.shape)
+    return f
+
+
+def random_inflow(setup, seed=2):
+    rng = np.random.default_rng(seed)
+    nb = setup.n_bfaces
+    inflow = np.zeros((nb, 6))
+    inflow[:, 0] = rng.uniform(0.5, 1.5, nb)
+    inflow[:, 1] = rng.uniform(0.8, 1.2, nb)
+    inflow[:, 2:5] = rng.uniform(-0.2, 0.2, (nb, 3))
+    inflow[:, 5] = rng.uniform(0.8, 1.2, nb)
+    return inflow
+
+
+def rel_err(a, b, floor=1e-300):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    return float(np.max(np.abs(a - b)) / max(scale, floor)) if a.size else 0.0

@@ -1,0 +1,175 @@
+"""GPU parity tests: the HIP backend (through the C ABI) against the CPU oracle on identical inputs.
+
+FP64 (the drop-in form): every backend method within 1e-12 relative (the only differences are the
+order of FP64 reductions over velocities and last-ulp differences of exp), and whole CIS iterations
+within 1e-10 relative (the bar the reference's own GPU-vs-CPU test uses with default settings,
+tests/kinetic/test_kinetic_gpu.cpp:133-158). FP32 f: within 2e-5 relative of the FP32 oracle (the
+reference's FP32 tolerance). Also: determinism and velocity-chunk independence, bitwise."""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import box_case, perturbed_distribution, random_inflow, rel_err
+from test_oracle import OpsCaller, exercise
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class HipCaller:
+    """The HIP Backend with the method names exercise() uses."""
+
+    def __init__(self, b):
+        self.b = b
+
+    def set_distribution(self, f):
+        self.b.set_distribution(f)
+
+    def get_distribution(self):
+        return self.b.get_distribution().ravel()
+
+    def moments(self):
+        return self.b.compute_moments()
+
+    def update_cell_params(self):
+        self.b.update_cell_params()
+
+    def set_inflow(self, inflow):
+        self.b.set_inflow(inflow)
+
+    def boundary_sums(self):
+        return self.b.boundary_sums()
+
+    def sweep(self):
+        return self.b.sweep()
+
+    def set_equilibrium(self, st):
+        self.b.set_equilibrium(st)
+
+    def gsis_correction(self, wh, wn):
+        return self.b.gsis_correction(wh, wn)
+
+    def face_flux(self, cells, entries):
+        return self.b.internal_face_mass_flux(cells, entries)
+
+
+METHOD_CASES = {
+    "box_mixed_fp64": (dict(kinetic={"precision": "fp64"}), 1e-12),
+    "box_mixed_fp64_rcm_3blocks_all": (dict(kinetic={"precision": "fp64", "sweep_blocks": 3, "cell_ordering": "rcm",
+                                                     "gsis_negative_check": "all", "limiter_eps": "absolute"}), 1e-12),
+    "box_mixed_fp64_dir13_limit_chunked": (dict(kinetic={"precision": "fp64", "equilibrium": "continuous",
+                                                         "gsis_negative_check": "limit", "sweep_order": "dir13",
+                                                         "positivity_limiter": True, "block_passes": 3,
+                                                         "velocity_chunk": 64}), 1e-12),
+    "box_mixed_fp32": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}), 2e-5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(METHOD_CASES))
+def test_backend_methods_match_oracle(gsk, meshgen, oracle_mod, tmp_path, name):
+    kw, tol = METHOD_CASES[name]
+    print(f"[gpu] backend methods {name}", flush=True)
+    setup = gsk.Setup(box_case(meshgen, str(tmp_path), **kw))
+    f = perturbed_distribution(setup)
+    inflow = random_inflow(setup)
+    cpu = exercise(OpsCaller(oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps), setup), setup, f, inflow)
+    gpu = exercise(HipCaller(gsk.Backend(setup)), setup, f, inflow)
+    for k in cpu:
+        if k == "gsis":
+            assert gpu[k][0] == cpu[k][0] and np.array_equal(gpu[k][1], cpu[k][1])
+        else:
+            e = rel_err(gpu[k], cpu[k])
+            assert e <= tol, (k, e)
+
+
+def _run_solver(gsk, setup, ops, iterations, snapshot_every=None):
+    sv = gsk.Solver(setup, ops) if ops is not None else gsk.Solver(setup)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    hist = []
+    for it in range(iterations):
+        sv.prepare()
+        info = sv.sweep()
+        hist.append((info["residual"], info["epsilon"]))
+    out = dict(moments=sv.moments(), hist=np.array(hist), flows=sv.patch_mass_flow(),
+               f=sv.distribution(), fluxes=sv.boundary_fluxes())
+    sv.close()
+    return out
+
+
+SOLVER_CASES = {
+    "box_mixed": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64"}),
+    "box_closed": lambda mg, d: box_case(mg, d, bcs="closed", kinetic={"precision": "fp64"}),
+    "box_vacuum_3blocks": lambda mg, d: box_case(mg, d, bcs="vacuum", kinetic={"precision": "fp64", "sweep_blocks": 3}),
+    "tube_curved_maxwell": lambda mg, d: mg.tube_case(d, n_core=4, n_ring=2, n_axial=3, n_vel=8, vrange=5.0, delta=5.0,
+                                                     accommodation=0.9, kinetic={"precision": "fp64"}),
+    "slab_200_24cubed": lambda mg, d: mg.slab_case(d, kinetic={"precision": "fp64"}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SOLVER_CASES))
+def test_solver_iterations_match_oracle(gsk, meshgen, oracle_mod, tmp_path, name):
+    print(f"[gpu] solver iterations {name}", flush=True)
+    setup = gsk.Setup(SOLVER_CASES[name](meshgen, str(tmp_path)))
+    its = 3 if name.startswith("slab") else 6
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+    cpu = _run_solver(gsk, setup, cb.ops, its)
+    gpu = _run_solver(gsk, setup, None, its)
+    assert rel_err(gpu["moments"], cpu["moments"]) <= 1e-10
+    assert rel_err(gpu["f"], cpu["f"]) <= 1e-10
+    assert rel_err(gpu["hist"][:, 0], cpu["hist"][:, 0]) <= 1e-10
+    for a, b in zip(gpu["fluxes"], cpu["fluxes"]):
+        assert rel_err(a, b) <= 1e-10
+    assert np.allclose(gpu["flows"], cpu["flows"], rtol=1e-10, atol=1e-12 * max(1.0, np.max(np.abs(cpu["flows"]))))
+
+
+def test_slab_benchmark_300_iterations_match_golden(gsk, meshgen, tmp_path):
+    """The 1D slab benchmark (200 cells, 24^3 on [-6, 6], delta = 1, 300 CIS iterations, FP64) against
+    the committed oracle vectors (tests/golden/slab_c1.npz): moments within 1e-10 relative."""
+    g = np.load(os.path.join(HERE, "golden", "slab_c1.npz"))
+    n = int(g["iterations"])
+    print(f"[gpu] slab benchmark {n} iterations vs golden", flush=True)
+    setup = gsk.Setup(meshgen.slab_case(str(tmp_path), kinetic={"precision": "fp64"}))
+    sv = gsk.Solver(setup)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    hist = []
+    for it in range(1, n + 1):
+        sv.prepare()
+        info = sv.sweep()
+        hist.append((info["residual"], info["epsilon"]))
+        if it == 1:
+            assert rel_err(sv.moments(), g["m1"]) <= 1e-12
+        if it == 10:
+            assert rel_err(sv.moments(), g["m10"]) <= 1e-11
+    m = sv.moments()
+    assert rel_err(m, g["mN"]) <= 1e-10
+    for k in (0, 1, 4):  # rho, u_x, T individually
+        assert rel_err(m[:, k], g["mN"][:, k]) <= 1e-10, k
+    assert rel_err(np.array(hist)[:, 0], g["history"][:, 0]) <= 1e-9
+    fp, fm, _ = sv.boundary_fluxes()
+    assert rel_err(fp, g["f_plus"]) <= 1e-10 and rel_err(fm, g["f_minus"]) <= 1e-10
+    sv.close()
+
+
+def test_fp32_form_tracks_fp32_oracle(gsk, meshgen, oracle_mod, tmp_path):
+    print("[gpu] fp32 form vs fp32 oracle", flush=True)
+    setup = gsk.Setup(box_case(meshgen, str(tmp_path), kinetic={"precision": "fp32"}))
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+    cpu = _run_solver(gsk, setup, cb.ops, 5)
+    gpu = _run_solver(gsk, setup, None, 5)
+    assert rel_err(gpu["moments"], cpu["moments"]) <= 2e-5
+
+
+def test_deterministic_and_chunk_independent(gsk, meshgen, tmp_path):
+    print("[gpu] determinism and chunk independence", flush=True)
+    path = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"})
+    runs = []
+    for chunk in (0, 0, 64):
+        setup = gsk.Setup(path, {"velocity_chunk": chunk})
+        runs.append(_run_solver(gsk, setup, None, 4))
+    for r in runs[1:]:
+        assert np.array_equal(r["moments"], runs[0]["moments"])
+        assert np.array_equal(r["f"], runs[0]["f"])
