@@ -103,6 +103,7 @@ void KineticSolver::correct_inflow(gsk_inflow_state& in) const {
 }
 
 void KineticSolver::init_static_inflow() {  // (:117-142)
+  wall_cache_patch_ = -1;
   for (Index b = 0; b < s_.n_bfaces(); ++b) {
     const PatchCondition& pc = patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])];
     gsk_inflow_state& in = inflow_[static_cast<std::size_t>(b)];
@@ -113,7 +114,14 @@ void KineticSolver::init_static_inflow() {  // (:117-142)
       in.uz = pc.u_wall.z;
       in.T = pc.T;
       in.scale = 1.0;
-      correct_inflow(in);
+      // every face of a wall patch has the same unit-density state: correct it once per patch
+      const int pi = s_.bface_patch[static_cast<std::size_t>(b)];
+      if (wall_cache_patch_ != pi) {
+        wall_cache_ = in;
+        correct_inflow(wall_cache_);
+        wall_cache_patch_ = pi;
+      }
+      in = wall_cache_;
     } else if (pc.kind == kBcPressure) {
       in.scale = 1.0;
       in.rho = pc.p / pc.T;
@@ -154,6 +162,7 @@ void KineticSolver::update_pressure_inflow() {  // (:172-260)
   const Index n_int = s_.mesh.n_internal_faces();
   const double K = s_.params.limiter_k;
   const bool limit = s_.limiter_mode != 0;
+  std::vector<Index> pending;
   for (const PatchCondition& pc : patches_) {
     if (pc.kind != kBcPressure) continue;
     for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) {
@@ -234,9 +243,12 @@ void KineticSolver::update_pressure_inflow() {  // (:172-260)
       in.uy = u.y;
       in.uz = u.z;
       in.T = T_b;
-      correct_inflow(in);
+      pending.push_back(b);
     }
   }
+  // discrete-equilibrium correction of the new inflow states: independent per face
+#pragma omp parallel for schedule(dynamic, 4)
+  for (std::size_t i = 0; i < pending.size(); ++i) correct_inflow(inflow_[static_cast<std::size_t>(pending[i])]);
 }
 
 std::array<double, 10> KineticSolver::compute_wall_unit_sums(Index b) const {  // (:263-277)

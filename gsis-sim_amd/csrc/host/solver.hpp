@@ -65,6 +65,8 @@ class KineticSolver {
   bool has_face_state_ = false;
   std::vector<double> residual_cell_, inner_residual_;
   double mass_target_ = 0.0;
+  gsk_inflow_state wall_cache_{};
+  int wall_cache_patch_ = -1;
   bool has_mass_target_ = false;
 };
 

@@ -353,94 +353,118 @@ __global__ void k_residual_reduce(const double* __restrict__ partial, int tiles,
   out[c] += s;
 }
 
-// LU-SGS forward sweep: one thread per (spatial block, chunk velocity) walks the block's cells in
-// its direction's order (sweep_forward_cell, kinetic_kernels.hpp:290-324). cur holds dF* of this
-// pass, prev the previous pass' dF (nullptr on the first pass: beta diagonal compensation).
+// ---- LU-SGS sweeps, level scheduled -------------------------------------------------------
+// The reference walks each spatial block's cells one after another per velocity (one thread per
+// (block, velocity), kinetic_cuda_kernels.cu:383-425). Here the cells of a block are grouped by
+// dependency level for each sweep direction: a cell's level is one more than the highest level of
+// the in-block neighbours that precede it in that direction's order (sweep_pos). Every cell then
+// reads only neighbours of lower levels, which are final, so processing the levels in turn gives
+// exactly the sequential result. One workgroup owns (block, direction, 64-velocity tile); its waves
+// share the cells of a level (lanes = velocities, coalesced) and meet at a barrier between levels.
+constexpr int kMaxDirs = 16;
+struct VelRange {
+  int beg[kMaxDirs];
+  int end[kMaxDirs];
+};
+
+struct LevelTables {
+  const int* cells;     // cells in (direction, block, level) order
+  const int* ptr;       // level boundaries; each (direction, block) group ends with a sentinel
+  const int* gb_begin;  // first ptr index of each (direction, block) group, plus the end
+  const int* vel_list;  // storage positions grouped by direction
+};
+
 template <class F>
-__global__ void k_sweep_forward(View<F> v, int k0, int kc, int stride, int n_blocks, const int* __restrict__ block_off,
-                                const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
-  const i64 lane = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x;
-  if (lane >= static_cast<i64>(n_blocks) * kc) return;
-  const int b = static_cast<int>(lane / kc), t = static_cast<int>(lane % kc);
-  const int p = k0 + t;
+__global__ void k_sweep_forward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
+                                LevelTables lt, VelRange vr, const F* __restrict__ rres, F* cur,
+                                const F* __restrict__ prev) {
+  const int g = blockIdx.y, b = blockIdx.z;
+  const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
+  if (vb >= vr.end[g]) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const bool valid = vb + lane < vr.end[g];
+  const int p = lt.vel_list[valid ? vb + lane : vb];
+  const int t = p - k0;
   const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
   const int c0 = block_off[b], c1 = block_off[b + 1];
-  const int* order = nullptr;
-  const int* pos = nullptr;
-  if (v.n_sweep_dirs > 0) {
-    const i64 off = static_cast<i64>(v.sweep_dir_of[p]) * v.n_cells;
-    order = v.sweep_order + off;
-    pos = v.sweep_pos + off;
-  }
-  for (int k = c0; k < c1; ++k) {
-    const int cell = order ? order[k] : k;
-    F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
-    const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-    const int pi = pos ? pos[cell] : cell;
-    for (int q = e0; q < e1; ++q) {
-      const CsrE<F>& e = v.csr[q];
-      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
-      const F a = x * e.sx + y * e.sy + z * e.sz;
-      if (a > F(0)) {
-        ap += a;
-      } else if (e.other >= 0) {
-        if (e.other < c0 || e.other >= c1) {
-          if (prev != nullptr)
-            Lout += a * prev[static_cast<i64>(e.other) * stride + t];
-          else
-            am_out -= a;
-        } else if ((pos ? pos[e.other] : e.other) < pi) {
-          L += a * cur[static_cast<i64>(e.other) * stride + t];
+  const int* pos = v.n_sweep_dirs > 0 ? v.sweep_pos + static_cast<i64>(g) * v.n_cells : nullptr;
+  const int l0 = lt.gb_begin[g * n_blocks + b], l1 = lt.gb_begin[g * n_blocks + b + 1] - 1;
+  for (int l = l0; l < l1; ++l) {
+    const int i1 = lt.ptr[l + 1];
+    for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
+      const int cell = lt.cells[i];
+      if (!valid) continue;
+      F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
+      const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+      const int pi = pos ? pos[cell] : cell;
+      for (int q = e0; q < e1; ++q) {
+        const CsrE<F>& e = v.csr[q];
+        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+        const F a = x * e.sx + y * e.sy + z * e.sz;
+        if (a > F(0)) {
+          ap += a;
+        } else if (e.other >= 0) {
+          if (e.other < c0 || e.other >= c1) {
+            if (prev != nullptr)
+              Lout += a * prev[static_cast<i64>(e.other) * stride + t];
+            else
+              am_out -= a;
+          } else if ((pos ? pos[e.other] : e.other) < pi) {
+            L += a * cur[static_cast<i64>(e.other) * stride + t];
+          }
         }
       }
+      const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+      const i64 wi = static_cast<i64>(cell) * stride + t;
+      cur[wi] = (-rres[wi] - Lout - L) / D;
     }
-    const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
-    const i64 wi = static_cast<i64>(cell) * stride + t;
-    cur[wi] = (-rres[wi] - Lout - L) / D;
+    __syncthreads();
   }
 }
 
-// LU-SGS backward sweep (sweep_backward_cell, kinetic_kernels.hpp:328-358); f += dF when update_f
 template <class F>
-__global__ void k_sweep_backward(View<F> v, int k0, int kc, int stride, int n_blocks, const int* __restrict__ block_off,
-                                 F* cur, int has_prev, int update_f) {
-  const i64 lane = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x;
-  if (lane >= static_cast<i64>(n_blocks) * kc) return;
-  const int b = static_cast<int>(lane / kc), t = static_cast<int>(lane % kc);
-  const int p = k0 + t;
+__global__ void k_sweep_backward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
+                                 LevelTables lt, VelRange vr, F* cur, int has_prev, int update_f) {
+  const int g = blockIdx.y, b = blockIdx.z;
+  const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
+  if (vb >= vr.end[g]) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const bool valid = vb + lane < vr.end[g];
+  const int p = lt.vel_list[valid ? vb + lane : vb];
+  const int t = p - k0;
   const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
   const int c0 = block_off[b], c1 = block_off[b + 1];
-  const int* order = nullptr;
-  const int* pos = nullptr;
-  if (v.n_sweep_dirs > 0) {
-    const i64 off = static_cast<i64>(v.sweep_dir_of[p]) * v.n_cells;
-    order = v.sweep_order + off;
-    pos = v.sweep_pos + off;
-  }
-  for (int k = c1 - 1; k >= c0; --k) {
-    const int cell = order ? order[k] : k;
-    F ap = F(0), am_out = F(0), U = F(0);
-    const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-    const int pi = pos ? pos[cell] : cell;
-    for (int q = e0; q < e1; ++q) {
-      const CsrE<F>& e = v.csr[q];
-      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
-      const F a = x * e.sx + y * e.sy + z * e.sz;
-      if (a > F(0)) {
-        ap += a;
-      } else if (e.other >= 0) {
-        if (e.other < c0 || e.other >= c1) {
-          if (!has_prev) am_out -= a;
-        } else if ((pos ? pos[e.other] : e.other) > pi) {
-          U += a * cur[static_cast<i64>(e.other) * stride + t];
+  const int* pos = v.n_sweep_dirs > 0 ? v.sweep_pos + static_cast<i64>(g) * v.n_cells : nullptr;
+  const int l0 = lt.gb_begin[g * n_blocks + b], l1 = lt.gb_begin[g * n_blocks + b + 1] - 1;
+  for (int l = l0; l < l1; ++l) {
+    const int i1 = lt.ptr[l + 1];
+    for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
+      const int cell = lt.cells[i];
+      if (!valid) continue;
+      F ap = F(0), am_out = F(0), U = F(0);
+      const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+      const int pi = pos ? pos[cell] : cell;
+      for (int q = e0; q < e1; ++q) {
+        const CsrE<F>& e = v.csr[q];
+        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+        const F a = x * e.sx + y * e.sy + z * e.sz;
+        if (a > F(0)) {
+          ap += a;
+        } else if (e.other >= 0) {
+          if (e.other < c0 || e.other >= c1) {
+            if (!has_prev) am_out -= a;
+          } else if ((pos ? pos[e.other] : e.other) > pi) {
+            U += a * cur[static_cast<i64>(e.other) * stride + t];
+          }
         }
       }
+      const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+      const i64 wi = static_cast<i64>(cell) * stride + t;
+      const F df = cur[wi] - U / D;
+      cur[wi] = df;
+      if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
     }
-    const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
-    const i64 wi = static_cast<i64>(cell) * stride + t;
-    const F df = cur[wi] - U / D;
-    cur[wi] = df;
-    if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
+    __syncthreads();
   }
 }
 
@@ -671,6 +695,7 @@ class HipBackend final : public Backend {
       sweep_pos_.upload(s.sweep_pos, static_cast<std::size_t>(n_sweep_dirs_) * nc_, stream_);
       sweep_dir_of_.upload(s.sweep_dir_of, static_cast<std::size_t>(nv_), stream_);
     }
+    build_levels(s);
     // Maxwell walls
     n_mw_ = s.n_maxwell_faces;
     if (n_mw_ > 0) {
@@ -878,19 +903,32 @@ class HipBackend final : public Backend {
       dev::k_residual_reduce<<<(nc_ + 255) / 256, 256, 0, stream_>>>(partial_.p, tiles_, nc_, residual_.p);
       HIPCHK(hipGetLastError());
       end();
-      const long long lanes = static_cast<long long>(n_blocks_) * ch.kc;
-      const unsigned sgrid = static_cast<unsigned>((lanes + 63) / 64);
+      // velocity tiles of every sweep direction inside this chunk
+      dev::VelRange vr{};
+      int max_tiles = 0;
+      for (int g = 0; g < n_dirs_eff_; ++g) {
+        const int* lo = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g)];
+        const int* hi = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g) + 1];
+        vr.beg[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0) - vel_list_host_.data());
+        vr.end[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0 + ch.kc) - vel_list_host_.data());
+        max_tiles = std::max(max_tiles, (vr.end[g] - vr.beg[g] + 63) / 64);
+      }
+      const dim3 sgrid(static_cast<unsigned>(std::max(max_tiles, 1)), static_cast<unsigned>(n_dirs_eff_),
+                       static_cast<unsigned>(n_blocks_));
+      const dev::LevelTables fw{fw_cells_.p, fw_ptr_.p, fw_gb_.p, vel_list_.p};
+      const dev::LevelTables bw{bw_cells_.p, bw_ptr_.p, bw_gb_.p, vel_list_.p};
       F* cur = bufA_.p;
       F* prev = nullptr;
       for (int pass = 0; pass < block_passes_; ++pass) {
         begin(kForward);
-        dev::k_sweep_forward<F><<<sgrid, 64, 0, stream_>>>(v, ch.k0, ch.kc, ch.stride, n_blocks_, block_off_.p, rres_.p, cur,
-                                                           prev);
+        dev::k_sweep_forward<F><<<sgrid, 64 * waves_fw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, fw,
+                                                                        vr, rres_.p, cur, prev);
         HIPCHK(hipGetLastError());
         end();
         begin(kBackward);
-        dev::k_sweep_backward<F><<<sgrid, 64, 0, stream_>>>(v, ch.k0, ch.kc, ch.stride, n_blocks_, block_off_.p, cur,
-                                                            prev != nullptr ? 1 : 0, pass + 1 == block_passes_ ? 1 : 0);
+        dev::k_sweep_backward<F><<<sgrid, 64 * waves_bw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, bw,
+                                                                         vr, cur, prev != nullptr ? 1 : 0,
+                                                                         pass + 1 == block_passes_ ? 1 : 0);
         HIPCHK(hipGetLastError());
         end();
         prev = cur;
@@ -961,6 +999,85 @@ class HipBackend final : public Backend {
   }
 
  private:
+  // Dependency levels of every (direction, block) for the forward and the backward sweep, and the
+  // storage positions of each direction's velocities (see k_sweep_forward).
+  void build_levels(const gsk_setup_view& s) {
+    n_dirs_eff_ = std::max(1, s.n_sweep_dirs);
+    GSK_CHECK(n_dirs_eff_ <= dev::kMaxDirs, "too many sweep directions");
+    const int nc = nc_, P = n_blocks_;
+    std::vector<int> fw_cells, bw_cells, fw_ptr, bw_ptr, fw_gb, bw_gb;
+    fw_cells.reserve(static_cast<std::size_t>(n_dirs_eff_) * nc);
+    bw_cells.reserve(fw_cells.capacity());
+    std::vector<int> lev(static_cast<std::size_t>(nc), 0), ident(static_cast<std::size_t>(nc));
+    for (int c = 0; c < nc; ++c) ident[static_cast<std::size_t>(c)] = c;
+    long long total_levels = 0;
+    for (int g = 0; g < n_dirs_eff_; ++g) {
+      const int* order = s.n_sweep_dirs > 0 ? s.sweep_order + static_cast<long long>(g) * nc : ident.data();
+      const int* pos = s.n_sweep_dirs > 0 ? s.sweep_pos + static_cast<long long>(g) * nc : ident.data();
+      for (int b = 0; b < P; ++b) {
+        const int c0 = s.block_offsets[b], c1 = s.block_offsets[b + 1];
+        for (int dirn = 0; dirn < 2; ++dirn) {  // 0 forward, 1 backward
+          int nlev = 0;
+          for (int kk = 0; kk < c1 - c0; ++kk) {
+            const int k = dirn == 0 ? c0 + kk : c1 - 1 - kk;
+            const int c = order[k];
+            int L = 0;
+            for (int q = s.csr_offsets[c]; q < s.csr_offsets[c + 1]; ++q) {
+              const int j = s.csr[q].other;
+              if (j < c0 || j >= c1) continue;
+              const bool before = dirn == 0 ? pos[j] < pos[c] : pos[j] > pos[c];
+              if (before) L = std::max(L, lev[static_cast<std::size_t>(j)] + 1);
+            }
+            lev[static_cast<std::size_t>(c)] = L;
+            nlev = std::max(nlev, L + 1);
+          }
+          if (c1 == c0) nlev = 0;
+          std::vector<int> count(static_cast<std::size_t>(nlev) + 1, 0);
+          for (int k = c0; k < c1; ++k) ++count[static_cast<std::size_t>(lev[static_cast<std::size_t>(order[k])]) + 1];
+          for (int l = 0; l < nlev; ++l) count[static_cast<std::size_t>(l) + 1] += count[static_cast<std::size_t>(l)];
+          std::vector<int>& cells = dirn == 0 ? fw_cells : bw_cells;
+          std::vector<int>& ptr = dirn == 0 ? fw_ptr : bw_ptr;
+          std::vector<int>& gb = dirn == 0 ? fw_gb : bw_gb;
+          const int base = static_cast<int>(cells.size());
+          gb.push_back(static_cast<int>(ptr.size()));
+          for (int l = 0; l <= nlev; ++l) ptr.push_back(base + count[static_cast<std::size_t>(l)]);
+          cells.resize(cells.size() + static_cast<std::size_t>(c1 - c0));
+          std::vector<int> fill(count.begin(), count.end() - 1);
+          for (int kk = 0; kk < c1 - c0; ++kk) {
+            const int k = dirn == 0 ? c0 + kk : c1 - 1 - kk;
+            const int c = order[k];
+            cells[static_cast<std::size_t>(base + fill[static_cast<std::size_t>(lev[static_cast<std::size_t>(c)])]++)] = c;
+          }
+          if (dirn == 0) total_levels += nlev;
+        }
+      }
+    }
+    fw_gb.push_back(static_cast<int>(fw_ptr.size()));
+    bw_gb.push_back(static_cast<int>(bw_ptr.size()));
+    fw_cells_.upload(fw_cells.data(), fw_cells.size(), stream_);
+    bw_cells_.upload(bw_cells.data(), bw_cells.size(), stream_);
+    fw_ptr_.upload(fw_ptr.data(), fw_ptr.size(), stream_);
+    bw_ptr_.upload(bw_ptr.data(), bw_ptr.size(), stream_);
+    fw_gb_.upload(fw_gb.data(), fw_gb.size(), stream_);
+    bw_gb_.upload(bw_gb.data(), bw_gb.size(), stream_);
+    // velocities grouped by direction (ascending storage position inside a direction)
+    vel_list_host_.clear();
+    dir_off_.assign(1, 0);
+    for (int g = 0; g < n_dirs_eff_; ++g) {
+      for (int p = 0; p < nv_; ++p)
+        if (s.n_sweep_dirs == 0 || s.sweep_dir_of[p] == g) vel_list_host_.push_back(p);
+      dir_off_.push_back(static_cast<int>(vel_list_host_.size()));
+    }
+    GSK_CHECK(static_cast<int>(vel_list_host_.size()) == nv_, "sweep direction table does not cover the velocities");
+    vel_list_.upload(vel_list_host_.data(), vel_list_host_.size(), stream_);
+    // waves per workgroup ~ mean level width (cells a level offers to run side by side)
+    const double width = total_levels > 0 ? static_cast<double>(n_dirs_eff_) * nc / static_cast<double>(total_levels) : 1.0;
+    int w = 1;
+    while (w < 8 && 2 * w <= width) w *= 2;
+    waves_fw_ = waves_bw_ = w;
+    mean_level_width_ = width;
+  }
+
   struct EventPair {
     hipEvent_t a, b;
     int phase;
@@ -1094,6 +1211,10 @@ class HipBackend final : public Backend {
   DBuf<double> moments_, shift_, residual_, bsums_, partial_;
   DBuf<int> flags_;
   DBuf<F> gx_, gy_, gz_, rres_, bufA_, bufB_;
+  int n_dirs_eff_ = 1, waves_fw_ = 1, waves_bw_ = 1;
+  double mean_level_width_ = 1.0;
+  DBuf<int> fw_cells_, bw_cells_, fw_ptr_, bw_ptr_, fw_gb_, bw_gb_, vel_list_;
+  std::vector<int> vel_list_host_, dir_off_;
   std::vector<int> deq_flags_;
   std::vector<int> host_bface_kind_;
   void* pinned_ = nullptr;
