@@ -94,6 +94,39 @@ def cpu_baseline(g, case_dir, tube_args, precision, iterations=3):
                        f"{iterations} CIS iterations after 1 untimed ({dt:.1f} s)")
 
 
+def time_to_mdot(g, mg, args):
+    """Metric 2: wall time until the inlet Mdot is within 1e-4 relative of the converged FP64 Mdot,
+    for the FP64 form and the FP32 form, on the 40k-cell tube (24^3, delta_rp = 10)."""
+    import gsis_sim_amd.cis_run as cr
+
+    tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
+    runs = {}
+    with tempfile.TemporaryDirectory() as d:
+        for prec in ("fp64", "fp32"):
+            path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=10.0, kinetic={"precision": prec}, name=f"tube_{prec}",
+                                **tube_args)
+            setup = g.Setup(path)
+            sv = g.Solver(setup)
+            log(f"time-to-Mdot: {prec}")
+            h = cr.run(sv, setup.mass_flow_factor(), inlet=0, outlet=1, residual_target=args.residual_target,
+                       max_iterations=args.max_iterations, log=lambda m, p=prec: log(f"{p}: {m}"))
+            sv.close()
+            runs[prec] = h
+    ref = float(runs["fp64"]["mdot_in"][-1])
+    out = {"metric": "time-to-Mdot (1e-4 rel)", "unit": "s", "workload": "tube40k_24cubed", "mdot_ref_fp64": ref,
+           "residual_target": args.residual_target}
+    for prec, h in runs.items():
+        t = cr.time_to_within(h, ref, 1e-4)
+        out[prec] = {"iterations": int(h["iteration"][-1]) + 1, "stop": h["reason"], "final_residual": float(h["residual"][-1]),
+                     "mdot_in": float(h["mdot_in"][-1]), "mdot_out": float(h["mdot_out"][-1]),
+                     "rel_err_vs_fp64": abs(float(h["mdot_in"][-1]) - ref) / abs(ref),
+                     "wall_s": float(h["t_wall"][-1]), "s_per_iteration": float(h["t_wall"][-1]) / len(h["t_wall"]),
+                     "time_to_mdot_s": None if t is None else t[0], "iterations_to_mdot": None if t is None else t[1]}
+    if out["fp32"]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
+        out["speedup_fp32_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out["fp32"]["time_to_mdot_s"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +134,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tube40k_24cubed_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--time-to-mdot", action="store_true",
+                    help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
+                         "within 1e-4 of the converged FP64 value")
+    ap.add_argument("--max-iterations", type=int, default=800)
+    ap.add_argument("--residual-target", type=float, default=1e-6)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device-to-device copy of 2 GiB (5 times) to calibrate PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the kinetic step is built for one MI355X in this version "
@@ -109,9 +149,21 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "no GPU"
+    if args.calibrate_copy:
+        n = 1 << 28  # 2 GiB of float64
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5}))
+        return
     g = _package()
     import gsis_sim_amd.meshgen as mg
 
+    if args.time_to_mdot:
+        time_to_mdot(g, mg, args)
+        return
     tube_args, nvel, precision = WORKLOADS[args.workload]
     s_bytes = 8 if precision == "fp64" else 4
     with tempfile.TemporaryDirectory() as d:

@@ -28,6 +28,13 @@ class Backend {
                                     std::size_t n_flags) = 0;
   virtual void internal_face_mass_flux(const int* cells, const int* entries, std::size_t n, double* out) = 0;
   virtual gsk_timings take_timings() = 0;
+  // boundary sums of at least the listed faces, written at their rows of out (20 x n_bfaces); the
+  // other rows hold unspecified values. Default: all faces.
+  virtual void boundary_sums_subset(const int* faces, std::size_t n_faces, double* out, std::size_t n) {
+    (void)faces;
+    (void)n_faces;
+    boundary_sums(out, n);
+  }
   virtual gsk_deq_counts deq_counts() = 0;
 };
 

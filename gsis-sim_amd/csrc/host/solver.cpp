@@ -428,7 +428,15 @@ const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
 }
 
 void KineticSolver::refresh_wall_inflow() {  // (:456-480)
-  backend_->boundary_sums(bsums_.data(), bsums_.size());
+  // only the wall faces' sums are read here
+  if (wall_faces_.empty()) {
+    for (Index b = 0; b < s_.n_bfaces(); ++b) {
+      const int k = patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])].kind;
+      if (k == kBcDiffuseWall || k == kBcMaxwellWall) wall_faces_.push_back(b);
+    }
+  }
+  if (wall_faces_.empty()) return;
+  backend_->boundary_sums_subset(wall_faces_.data(), wall_faces_.size(), bsums_.data(), bsums_.size());
   bool any = false;
   for (Index b = 0; b < s_.n_bfaces(); ++b) {
     const auto bi = static_cast<std::size_t>(b);

@@ -66,6 +66,7 @@ class KineticSolver {
   std::vector<double> residual_cell_, inner_residual_;
   double mass_target_ = 0.0;
   gsk_inflow_state wall_cache_{};
+  std::vector<int> wall_faces_;
   int wall_cache_patch_ = -1;
   bool has_mass_target_ = false;
 };

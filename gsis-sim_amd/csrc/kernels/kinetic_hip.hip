@@ -299,23 +299,27 @@ __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int*
   }
 }
 
-// XCD-aware (cell, velocity tile) mapping: workgroups are dealt round-robin to the 8 XCDs, so
-// give XCD x a contiguous range of cells; neighbouring cells (close in the RCB/RCM numbering)
-// then share one L2 when their f rows are re-read as upwind / stencil values.
-__device__ __forceinline__ bool xcd_tile(i64 n_items, int tiles, int& cell, int& tile) {
-  const i64 bid = blockIdx.x;
-  const i64 per = (static_cast<i64>(gridDim.x) + 7) / 8;
-  const i64 logical = (bid % 8) * per + bid / 8;
-  if (logical >= n_items) return false;
-  cell = static_cast<int>(logical / tiles);
-  tile = static_cast<int>(logical % tiles);
+// XCD-aware (cell, velocity tile) mapping. Workgroups are dealt round-robin to the 8 XCDs; XCD x
+// takes the velocity tiles t = x (mod 8) of every cell, cells in order. The f / gradient rows that a
+// cell re-reads from its neighbours (close in the RCB/RCM numbering) were then touched by the same
+// XCD a few hundred workgroups earlier, with only an eighth of the velocities in between, so they
+// are still in that XCD's L2. Grid: 8 * ceil(tiles / 8) * n_cells workgroups.
+__device__ __forceinline__ bool xcd_tile(int n_cells, int tiles, int& cell, int& tile) {
+  const int x = static_cast<int>(blockIdx.x % 8);
+  const i64 k = blockIdx.x / 8;
+  const int ntx = (tiles - x + 7) / 8;  // tiles owned by this XCD
+  if (ntx <= 0) return false;
+  const i64 c = k / ntx;
+  if (c >= n_cells) return false;
+  cell = static_cast<int>(c);
+  tile = x + 8 * static_cast<int>(k % ntx);
   return true;
 }
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles) {
   int c, tile;
-  if (!xcd_tile(static_cast<i64>(v.n_cells) * tiles, tiles, c, tile)) return;
+  if (!xcd_tile(v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   if (t >= ch.kc) return;
   const int p = ch.k0 + t;
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int
                                                      double* __restrict__ partial) {
   __shared__ double lds[kWaves];
   int c, tile;
-  if (!xcd_tile(static_cast<i64>(v.n_cells) * tiles, tiles, c, tile)) return;
+  if (!xcd_tile(v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   double r2[1] = {0.0};
   if (t < ch.kc) {
@@ -367,6 +371,18 @@ struct VelRange {
   int end[kMaxDirs];
 };
 
+// One cell of one sweep (forward or backward) for one direction, in level order: its faces in CSR
+// order with the class each has in that sweep, so the kernel needs no order / position / boundary
+// lookups. code = neighbour << 2 | class; class 0 skip (periodic), 1 contributes a+ only (boundary,
+// or an in-block neighbour on the downstream side), 2 in-block upstream neighbour, 3 out-of-block
+// neighbour. nf < 0: more than kGather faces, use the CSR path.
+template <class F>
+struct SweepRec {
+  int cell, nf;
+  int code[kGather];
+  F sx[kGather], sy[kGather], sz[kGather];
+};
+
 struct LevelTables {
   const int* cells;     // cells in (direction, block, level) order
   const int* ptr;       // level boundaries; each (direction, block) group ends with a sentinel
@@ -374,14 +390,225 @@ struct LevelTables {
   const int* vel_list;  // storage positions grouped by direction
 };
 
+// one cell of the forward sweep (sweep_forward_cell, kinetic_kernels.hpp:290-324); the upwind
+// neighbour values are gathered before the in-order accumulation
+template <class F>
+__device__ __forceinline__ void forward_cell(const View<F>& v, int cell, int t, int c0, int c1, const int* pos, F x, F y,
+                                             F z, int stride, const F* __restrict__ rres, F* cur,
+                                             const F* __restrict__ prev) {
+  F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
+  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+  const int pi = pos ? pos[cell] : cell;
+  const i64 wi = static_cast<i64>(cell) * stride + t;
+  const F Ri = rres[wi];
+  if (e1 - e0 <= kGather) {
+    F ak[kGather], nb[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        ak[k] = x * e.sx + y * e.sy + z * e.sz;
+        nb[k] = F(0);
+        if (e.other >= 0 && !(ak[k] > F(0))) {
+          const i64 wj = static_cast<i64>(e.other) * stride + t;
+          if (e.other < c0 || e.other >= c1) {
+            if (prev != nullptr) nb[k] = prev[wj];
+          } else if ((pos ? pos[e.other] : e.other) < pi) {
+            nb[k] = cur[wj];
+          }
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+        const F a = ak[k];
+        if (a > F(0)) {
+          ap += a;
+        } else if (e.other >= 0) {
+          if (e.other < c0 || e.other >= c1) {
+            if (prev != nullptr)
+              Lout += a * nb[k];
+            else
+              am_out -= a;
+          } else if ((pos ? pos[e.other] : e.other) < pi) {
+            L += a * nb[k];
+          }
+        }
+      }
+  } else {
+    for (int q = e0; q < e1; ++q) {
+      const CsrE<F>& e = v.csr[q];
+      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+      const F a = x * e.sx + y * e.sy + z * e.sz;
+      if (a > F(0)) {
+        ap += a;
+      } else if (e.other >= 0) {
+        if (e.other < c0 || e.other >= c1) {
+          if (prev != nullptr)
+            Lout += a * prev[static_cast<i64>(e.other) * stride + t];
+          else
+            am_out -= a;
+        } else if ((pos ? pos[e.other] : e.other) < pi) {
+          L += a * cur[static_cast<i64>(e.other) * stride + t];
+        }
+      }
+    }
+  }
+  const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+  cur[wi] = (-Ri - Lout - L) / D;
+}
+
+// one cell of the backward sweep (sweep_backward_cell, kinetic_kernels.hpp:328-358)
+template <class F>
+__device__ __forceinline__ void backward_cell(const View<F>& v, int cell, int t, int p, int c0, int c1, const int* pos,
+                                              F x, F y, F z, int stride, F* cur, int has_prev, int update_f) {
+  F ap = F(0), am_out = F(0), U = F(0);
+  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+  const int pi = pos ? pos[cell] : cell;
+  const i64 wi = static_cast<i64>(cell) * stride + t;
+  const F own = cur[wi];
+  if (e1 - e0 <= kGather) {
+    F ak[kGather], nb[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        ak[k] = x * e.sx + y * e.sy + z * e.sz;
+        nb[k] = F(0);
+        if (e.other >= c0 && e.other < c1 && !(ak[k] > F(0)) && (pos ? pos[e.other] : e.other) > pi)
+          nb[k] = cur[static_cast<i64>(e.other) * stride + t];
+      }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+        const F a = ak[k];
+        if (a > F(0)) {
+          ap += a;
+        } else if (e.other >= 0) {
+          if (e.other < c0 || e.other >= c1) {
+            if (!has_prev) am_out -= a;
+          } else if ((pos ? pos[e.other] : e.other) > pi) {
+            U += a * nb[k];
+          }
+        }
+      }
+  } else {
+    for (int q = e0; q < e1; ++q) {
+      const CsrE<F>& e = v.csr[q];
+      if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
+      const F a = x * e.sx + y * e.sy + z * e.sz;
+      if (a > F(0)) {
+        ap += a;
+      } else if (e.other >= 0) {
+        if (e.other < c0 || e.other >= c1) {
+          if (!has_prev) am_out -= a;
+        } else if ((pos ? pos[e.other] : e.other) > pi) {
+          U += a * cur[static_cast<i64>(e.other) * stride + t];
+        }
+      }
+    }
+  }
+  const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
+  const F df = own - U / D;
+  cur[wi] = df;
+  if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
+}
+
+// forward cell from a sweep record: same operations and order as forward_cell
+template <class F>
+__device__ __forceinline__ void forward_rec(const View<F>& v, const SweepRec<F>& r, int t, F x, F y, F z, int stride,
+                                            const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
+  const i64 wi = static_cast<i64>(r.cell) * stride + t;
+  const F Ri = rres[wi];
+  const F diag = v.cells[r.cell].diag;
+  F ak[kGather], nb[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < r.nf) {
+      const int cls = r.code[k] & 3;
+      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      nb[k] = F(0);
+      if (cls >= 2 && !(ak[k] > F(0))) {
+        const i64 wj = static_cast<i64>(r.code[k] >> 2) * stride + t;
+        if (cls == 2)
+          nb[k] = cur[wj];
+        else if (prev != nullptr)
+          nb[k] = prev[wj];
+      }
+    }
+  F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < r.nf) {
+      const int cls = r.code[k] & 3;
+      if (cls == 0) continue;
+      const F a = ak[k];
+      if (a > F(0)) {
+        ap += a;
+      } else if (cls == 3) {
+        if (prev != nullptr)
+          Lout += a * nb[k];
+        else
+          am_out -= a;
+      } else if (cls == 2) {
+        L += a * nb[k];
+      }
+    }
+  const F D = diag + v.diag_scale * ap + v.interface_damping * am_out;
+  cur[wi] = (-Ri - Lout - L) / D;
+}
+
+// backward cell from a sweep record: same operations and order as backward_cell
+template <class F>
+__device__ __forceinline__ void backward_rec(const View<F>& v, const SweepRec<F>& r, int t, int p, F x, F y, F z,
+                                             int stride, F* cur, int has_prev, int update_f) {
+  const i64 wi = static_cast<i64>(r.cell) * stride + t;
+  const F own = cur[wi];
+  const F diag = v.cells[r.cell].diag;
+  F ak[kGather], nb[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < r.nf) {
+      const int cls = r.code[k] & 3;
+      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      nb[k] = F(0);
+      if (cls == 2 && !(ak[k] > F(0))) nb[k] = cur[static_cast<i64>(r.code[k] >> 2) * stride + t];
+    }
+  F ap = F(0), am_out = F(0), U = F(0);
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < r.nf) {
+      const int cls = r.code[k] & 3;
+      if (cls == 0) continue;
+      const F a = ak[k];
+      if (a > F(0)) {
+        ap += a;
+      } else if (cls == 3) {
+        if (!has_prev) am_out -= a;
+      } else if (cls == 2) {
+        U += a * nb[k];
+      }
+    }
+  const F D = diag + v.diag_scale * ap + v.interface_damping * am_out;
+  const F df = own - U / D;
+  cur[wi] = df;
+  if (update_f) v.f[static_cast<i64>(r.cell) * v.nv + p] += df;
+}
+
 template <class F>
 __global__ void k_sweep_forward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
-                                LevelTables lt, VelRange vr, const F* __restrict__ rres, F* cur,
-                                const F* __restrict__ prev) {
+                                LevelTables lt, const SweepRec<F>* __restrict__ recs, VelRange vr,
+                                const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
   const int g = blockIdx.y, b = blockIdx.z;
   const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
   if (vb >= vr.end[g]) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int nwaves = blockDim.x >> 6;
   const bool valid = vb + lane < vr.end[g];
   const int p = lt.vel_list[valid ? vb + lane : vb];
   const int t = p - k0;
@@ -394,29 +621,11 @@ __global__ void k_sweep_forward(View<F> v, int k0, int stride, int n_blocks, con
     for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
       const int cell = lt.cells[i];
       if (!valid) continue;
-      F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
-      const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-      const int pi = pos ? pos[cell] : cell;
-      for (int q = e0; q < e1; ++q) {
-        const CsrE<F>& e = v.csr[q];
-        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
-        const F a = x * e.sx + y * e.sy + z * e.sz;
-        if (a > F(0)) {
-          ap += a;
-        } else if (e.other >= 0) {
-          if (e.other < c0 || e.other >= c1) {
-            if (prev != nullptr)
-              Lout += a * prev[static_cast<i64>(e.other) * stride + t];
-            else
-              am_out -= a;
-          } else if ((pos ? pos[e.other] : e.other) < pi) {
-            L += a * cur[static_cast<i64>(e.other) * stride + t];
-          }
-        }
-      }
-      const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
-      const i64 wi = static_cast<i64>(cell) * stride + t;
-      cur[wi] = (-rres[wi] - Lout - L) / D;
+      const SweepRec<F>& r = recs[i];
+      if (r.nf >= 0)
+        forward_rec(v, r, t, x, y, z, stride, rres, cur, prev);
+      else
+        forward_cell(v, cell, t, c0, c1, pos, x, y, z, stride, rres, cur, prev);
     }
     __syncthreads();
   }
@@ -424,11 +633,14 @@ __global__ void k_sweep_forward(View<F> v, int k0, int stride, int n_blocks, con
 
 template <class F>
 __global__ void k_sweep_backward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
-                                 LevelTables lt, VelRange vr, F* cur, int has_prev, int update_f) {
+                                 LevelTables lt, const SweepRec<F>* __restrict__ recs, VelRange vr, F* cur,
+                                 int has_prev, int update_f) {
   const int g = blockIdx.y, b = blockIdx.z;
   const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
   if (vb >= vr.end[g]) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int nwaves = blockDim.x >> 6;
   const bool valid = vb + lane < vr.end[g];
   const int p = lt.vel_list[valid ? vb + lane : vb];
   const int t = p - k0;
@@ -441,28 +653,11 @@ __global__ void k_sweep_backward(View<F> v, int k0, int stride, int n_blocks, co
     for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
       const int cell = lt.cells[i];
       if (!valid) continue;
-      F ap = F(0), am_out = F(0), U = F(0);
-      const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-      const int pi = pos ? pos[cell] : cell;
-      for (int q = e0; q < e1; ++q) {
-        const CsrE<F>& e = v.csr[q];
-        if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
-        const F a = x * e.sx + y * e.sy + z * e.sz;
-        if (a > F(0)) {
-          ap += a;
-        } else if (e.other >= 0) {
-          if (e.other < c0 || e.other >= c1) {
-            if (!has_prev) am_out -= a;
-          } else if ((pos ? pos[e.other] : e.other) > pi) {
-            U += a * cur[static_cast<i64>(e.other) * stride + t];
-          }
-        }
-      }
-      const F D = v.cells[cell].diag + v.diag_scale * ap + v.interface_damping * am_out;
-      const i64 wi = static_cast<i64>(cell) * stride + t;
-      const F df = cur[wi] - U / D;
-      cur[wi] = df;
-      if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
+      const SweepRec<F>& r = recs[i];
+      if (r.nf >= 0)
+        backward_rec(v, r, t, p, x, y, z, stride, cur, has_prev, update_f);
+      else
+        backward_cell(v, cell, t, p, c0, c1, pos, x, y, z, stride, cur, has_prev, update_f);
     }
     __syncthreads();
   }
@@ -886,14 +1081,33 @@ class HipBackend final : public Backend {
     download(out, bsums_.p, n);
   }
 
+  void boundary_sums_subset(const int* faces, std::size_t n_faces, double* out, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nb_) * dev::kBoundarySums, "boundary sums length differs from 20 x faces");
+    if (n_faces == 0) return;
+    for (std::size_t i = 0; i < n_faces; ++i) GSK_CHECK(faces[i] >= 0 && faces[i] < nb_, "face index out of range");
+    begin(kBoundary);
+    if (n_mw_ > 0) {
+      const dim3 grid((nv_ + 255) / 256, nb_);
+      dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
+      dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+    }
+    subset_.upload(faces, n_faces, stream_);
+    dev::k_boundary_sums<F><<<static_cast<unsigned>(std::min<std::size_t>(n_faces, 65536)), dev::kBlock, 0, stream_>>>(
+        view(), subset_.p, static_cast<int>(n_faces), bsums_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    download(out, bsums_.p, n);
+  }
+
   void sweep(double* residual_cell, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
     HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
     const dev::View<F> v = view();
     for (int k0 = 0; k0 < nv_; k0 += chunk_) {
       dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
-      const long long items = static_cast<long long>(nc_) * tiles_;
-      const unsigned grid = static_cast<unsigned>((items + 7) / 8 * 8);
+      const long long grid_ll = 8LL * ((tiles_ + 7) / 8) * nc_;
+      GSK_CHECK(grid_ll < (1LL << 32), "grid too large");
+      const unsigned grid = static_cast<unsigned>(grid_ll);
       begin(kGradient);
       dev::k_gradient<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_);
       HIPCHK(hipGetLastError());
@@ -922,12 +1136,12 @@ class HipBackend final : public Backend {
       for (int pass = 0; pass < block_passes_; ++pass) {
         begin(kForward);
         dev::k_sweep_forward<F><<<sgrid, 64 * waves_fw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, fw,
-                                                                        vr, rres_.p, cur, prev);
+                                                                        fw_recs_.p, vr, rres_.p, cur, prev);
         HIPCHK(hipGetLastError());
         end();
         begin(kBackward);
         dev::k_sweep_backward<F><<<sgrid, 64 * waves_bw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, bw,
-                                                                         vr, cur, prev != nullptr ? 1 : 0,
+                                                                         bw_recs_.p, vr, cur, prev != nullptr ? 1 : 0,
                                                                          pass + 1 == block_passes_ ? 1 : 0);
         HIPCHK(hipGetLastError());
         end();
@@ -1006,6 +1220,8 @@ class HipBackend final : public Backend {
     GSK_CHECK(n_dirs_eff_ <= dev::kMaxDirs, "too many sweep directions");
     const int nc = nc_, P = n_blocks_;
     std::vector<int> fw_cells, bw_cells, fw_ptr, bw_ptr, fw_gb, bw_gb;
+    std::vector<dev::SweepRec<F>> fw_recs, bw_recs;
+    GSK_CHECK(static_cast<long long>(nc_) * 4 < (1LL << 31), "cell count too large for sweep records");
     fw_cells.reserve(static_cast<std::size_t>(n_dirs_eff_) * nc);
     bw_cells.reserve(fw_cells.capacity());
     std::vector<int> lev(static_cast<std::size_t>(nc), 0), ident(static_cast<std::size_t>(nc));
@@ -1048,12 +1264,43 @@ class HipBackend final : public Backend {
             const int c = order[k];
             cells[static_cast<std::size_t>(base + fill[static_cast<std::size_t>(lev[static_cast<std::size_t>(c)])]++)] = c;
           }
+          std::vector<dev::SweepRec<F>>& recs = dirn == 0 ? fw_recs : bw_recs;
+          recs.resize(cells.size());
+          for (std::size_t i = static_cast<std::size_t>(base); i < cells.size(); ++i) {
+            const int c = cells[i];
+            dev::SweepRec<F>& r = recs[i];
+            std::memset(&r, 0, sizeof(r));
+            r.cell = c;
+            const int e0 = s.csr_offsets[c], e1 = s.csr_offsets[c + 1];
+            if (e1 - e0 > dev::kGather) {
+              r.nf = -1;
+              continue;
+            }
+            r.nf = e1 - e0;
+            for (int q = e0; q < e1; ++q) {
+              const gsk_csr_entry& e = s.csr[q];
+              const int k = q - e0;
+              int cls;
+              if (e.other < 0)
+                cls = s.bfaces[e.bface].kind == GSK_BC_PERIODIC ? 0 : 1;
+              else if (e.other < c0 || e.other >= c1)
+                cls = 3;
+              else
+                cls = (dirn == 0 ? pos[e.other] < pos[c] : pos[e.other] > pos[c]) ? 2 : 1;
+              r.code[k] = (e.other >= 0 ? e.other : 0) * 4 + cls;
+              r.sx[k] = static_cast<F>(e.s[0]);
+              r.sy[k] = static_cast<F>(e.s[1]);
+              r.sz[k] = static_cast<F>(e.s[2]);
+            }
+          }
           if (dirn == 0) total_levels += nlev;
         }
       }
     }
     fw_gb.push_back(static_cast<int>(fw_ptr.size()));
     bw_gb.push_back(static_cast<int>(bw_ptr.size()));
+    fw_recs_.upload(fw_recs.data(), fw_recs.size(), stream_);
+    bw_recs_.upload(bw_recs.data(), bw_recs.size(), stream_);
     fw_cells_.upload(fw_cells.data(), fw_cells.size(), stream_);
     bw_cells_.upload(bw_cells.data(), bw_cells.size(), stream_);
     fw_ptr_.upload(fw_ptr.data(), fw_ptr.size(), stream_);
@@ -1214,6 +1461,8 @@ class HipBackend final : public Backend {
   int n_dirs_eff_ = 1, waves_fw_ = 1, waves_bw_ = 1;
   double mean_level_width_ = 1.0;
   DBuf<int> fw_cells_, bw_cells_, fw_ptr_, bw_ptr_, fw_gb_, bw_gb_, vel_list_;
+  DBuf<dev::SweepRec<F>> fw_recs_, bw_recs_;
+  DBuf<int> subset_;
   std::vector<int> vel_list_host_, dir_off_;
   std::vector<int> deq_flags_;
   std::vector<int> host_bface_kind_;

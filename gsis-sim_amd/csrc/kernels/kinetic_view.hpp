@@ -96,22 +96,14 @@ __device__ F boundary_value_first_order(const View<F>& v, const CsrE<F>& e, int 
   return inflow_value(v.inflow[e.bface], x, y, z);
 }
 
-// limited LSQ gradient phi * grad f at (cell, p) (kinetic_kernels.hpp:165-201)
+// limited LSQ gradient phi * grad f at (cell, p) (kinetic_kernels.hpp:165-201).
+// The neighbour values are gathered first (all loads in flight together), then combined in CSR order,
+// so the arithmetic is the reference's while the wave waits on memory once instead of once per face.
+constexpr int kGather = 6;  // faces held in registers (hex 6, prism/pyramid 5, tet 4); more: CSR path
+
 template <class F>
-__device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
-  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-  F gx = F(0), gy = F(0), gz = F(0);
-  F fmx = fi, fmn = fi;
-  for (int k = e0; k < e1; ++k) {
-    const CsrE<F>& e = v.csr[k];
-    const F fk = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
-    const F d = fk - fi;
-    gx += e.cx * d;
-    gy += e.cy * d;
-    gz += e.cz * d;
-    fmx = dmax(fmx, fk);
-    fmn = dmin(fmn, fk);
-  }
+__device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, int e0, int e1, F fi, F fmx, F fmn, F gx,
+                                                F gy, F gz, F& ogx, F& ogy, F& ogz) {
   if (v.limiter_mode == kLimNone && v.positivity == 0) {
     ogx = gx;
     ogy = gy;
@@ -141,6 +133,46 @@ __device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F 
   ogx = phi * gx;
   ogy = phi * gy;
   ogz = phi * gz;
+}
+
+template <class F>
+__device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
+  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+  F gx = F(0), gy = F(0), gz = F(0);
+  F fmx = fi, fmn = fi;
+  if (e1 - e0 <= kGather) {
+    F fk[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const int o = v.csr[e0 + k].other;
+        fk[k] = o >= 0 ? f_at(v, o, p) : F(0);
+      }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        if (e.other < 0) fk[k] = boundary_value_first_order(v, e, cell, p, x, y, z, fi);
+        const F d = fk[k] - fi;
+        gx += e.cx * d;
+        gy += e.cy * d;
+        gz += e.cz * d;
+        fmx = dmax(fmx, fk[k]);
+        fmn = dmin(fmn, fk[k]);
+      }
+  } else {
+    for (int k = e0; k < e1; ++k) {
+      const CsrE<F>& e = v.csr[k];
+      const F fk = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
+      const F d = fk - fi;
+      gx += e.cx * d;
+      gy += e.cy * d;
+      gz += e.cz * d;
+      fmx = dmax(fmx, fk);
+      fmn = dmin(fmn, fk);
+    }
+  }
+  limit_and_scale(v, cell, e0, e1, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
 }
 
 // outgoing reconstructed value at a boundary face (kinetic_kernels.hpp:204-211)
@@ -222,7 +254,8 @@ __device__ F internal_face_flux_term(const View<F>& v, int cell, const CsrE<F>& 
   return a * ff;
 }
 
-// steady residual R = sum_e a f_e - coll (f_S - f_i) (kinetic_kernels.hpp:238-278)
+// steady residual R = sum_e a f_e - coll (f_S - f_i) (kinetic_kernels.hpp:238-278); upwind neighbour
+// values (f and its limited gradient) are gathered before the flux sum, which runs in CSR order.
 template <class F>
 __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int t) {
   const int p = ch.k0 + t;
@@ -232,30 +265,81 @@ __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int 
   const F gix = ch.gx[wi], giy = ch.gy[wi], giz = ch.gz[wi];
   F R = F(0);
   const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
-  for (int k = e0; k < e1; ++k) {
-    const CsrE<F>& e = v.csr[k];
-    int kind = -1;
-    if (e.other < 0) {
-      kind = v.bf[e.bface].kind;
-      if (kind == kBcPeriodic) continue;
+  if (e1 - e0 <= kGather) {
+    F ak[kGather], fu[kGather], gux[kGather], guy[kGather], guz[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        ak[k] = x * e.sx + y * e.sy + z * e.sz;
+        fu[k] = gux[k] = guy[k] = guz[k] = F(0);
+        if (ak[k] < F(0)) {
+          int src_cell = -1, src_t = t;
+          if (e.other >= 0) {
+            src_cell = e.other;
+          } else if (v.bf[e.bface].kind == kBcSymmetry) {
+            src_cell = cell;
+            src_t = t ^ v.bf[e.bface].mirror_xor;
+          }
+          if (src_cell >= 0) {
+            const i64 wj = static_cast<i64>(src_cell) * ch.stride + src_t;
+            fu[k] = f_at(v, src_cell, ch.k0 + src_t);
+            gux[k] = ch.gx[wj];
+            guy[k] = ch.gy[wj];
+            guz[k] = ch.gz[wj];
+          }
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k)
+      if (k < e1 - e0) {
+        const CsrE<F>& e = v.csr[e0 + k];
+        int kind = -1;
+        if (e.other < 0) {
+          kind = v.bf[e.bface].kind;
+          if (kind == kBcPeriodic) continue;
+        }
+        const F a = ak[k];
+        F ff;
+        if (a >= F(0)) {
+          ff = fi + gix * e.dsx + giy * e.dsy + giz * e.dsz;
+        } else if (e.other >= 0) {
+          ff = fu[k] + gux[k] * e.dox + guy[k] * e.doy + guz[k] * e.doz;
+        } else if (kind == kBcSymmetry) {
+          ff = fu[k] + gux[k] * e.dsx + guy[k] * e.dsy + guz[k] * e.dsz;
+        } else if (kind == kBcMaxwell) {
+          ff = maxwell_inflow(v, e.bface, p, x, y, z);
+        } else {
+          ff = inflow_value(v.inflow[e.bface], x, y, z);
+        }
+        R += a * ff;
+      }
+  } else {
+    for (int k = e0; k < e1; ++k) {
+      const CsrE<F>& e = v.csr[k];
+      int kind = -1;
+      if (e.other < 0) {
+        kind = v.bf[e.bface].kind;
+        if (kind == kBcPeriodic) continue;
+      }
+      const F a = x * e.sx + y * e.sy + z * e.sz;
+      F ff;
+      if (a >= F(0)) {
+        ff = fi + gix * e.dsx + giy * e.dsy + giz * e.dsz;
+      } else if (e.other >= 0) {
+        const i64 wj = static_cast<i64>(e.other) * ch.stride + t;
+        ff = f_at(v, e.other, p) + ch.gx[wj] * e.dox + ch.gy[wj] * e.doy + ch.gz[wj] * e.doz;
+      } else if (kind == kBcSymmetry) {
+        const int tm = t ^ v.bf[e.bface].mirror_xor;
+        const i64 wm = static_cast<i64>(cell) * ch.stride + tm;
+        ff = f_at(v, cell, ch.k0 + tm) + ch.gx[wm] * e.dsx + ch.gy[wm] * e.dsy + ch.gz[wm] * e.dsz;
+      } else if (kind == kBcMaxwell) {
+        ff = maxwell_inflow(v, e.bface, p, x, y, z);
+      } else {
+        ff = inflow_value(v.inflow[e.bface], x, y, z);
+      }
+      R += a * ff;
     }
-    const F a = x * e.sx + y * e.sy + z * e.sz;
-    F ff;
-    if (a >= F(0)) {
-      ff = fi + gix * e.dsx + giy * e.dsy + giz * e.dsz;
-    } else if (e.other >= 0) {
-      const i64 wj = static_cast<i64>(e.other) * ch.stride + t;
-      ff = f_at(v, e.other, p) + ch.gx[wj] * e.dox + ch.gy[wj] * e.doy + ch.gz[wj] * e.doz;
-    } else if (kind == kBcSymmetry) {
-      const int tm = t ^ v.bf[e.bface].mirror_xor;
-      const i64 wm = static_cast<i64>(cell) * ch.stride + tm;
-      ff = f_at(v, cell, ch.k0 + tm) + ch.gx[wm] * e.dsx + ch.gy[wm] * e.dsy + ch.gz[wm] * e.dsz;
-    } else if (kind == kBcMaxwell) {
-      ff = maxwell_inflow(v, e.bface, p, x, y, z);
-    } else {
-      ff = inflow_value(v.inflow[e.bface], x, y, z);
-    }
-    R += a * ff;
   }
   const CellP<F>& cp = v.cells[cell];
   R -= cp.coll * (target_eval(cp.s, x, y, z) - fi);
