@@ -62,6 +62,63 @@ def phase_bytes_per_pair(phase, passes):
     return None
 
 
+def measured_traffic(workload, kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for a plain 2 GiB copy),
+    or None when no summary for this kernel exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    name = f"k_{kernel}<{'double' if precision == 'fp64' else 'float'}>"
+    k = data.get("kernels", {}).get(name)
+    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
+
+
+def summarize_trace(stats_csv, workload, command):
+    """Markdown table of a rocprofv3 --kernel-trace --stats kernel summary (CSV output)."""
+    import csv
+
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = [f"# rocprofv3 kernel trace: {workload}", "", "Command (repo root on the GPU box):", "",
+             f"    {command}", "", "| kernel | calls | total ms | mean ms | % |", "|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"].split("(")[0].replace("void ", "").replace("gsk::dev::", "")
+        lines.append(f"| {name} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e6:.4f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(lines) + "\n"
+
+
+def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload, commands):
+    """HBM bytes per launch of each kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
+    dispatch), each scaled by the reading of the same counter for a plain 2 GiB device copy."""
+    import collections
+    import csv
+
+    def load(path):
+        d = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("gsk::dev::", "")
+            d[name].append(float(r["Counter_Value"]))
+        return d
+
+    f, w, cf, cw = (load(p) for p in (fetch_csv, write_csv, copy_fetch_csv, copy_write_csv))
+    copy_bytes = 2 * 1024 ** 3
+    copy = "__amd_rocclr_copyBuffer"
+    sf = copy_bytes / (sum(cf[copy]) / len(cf[copy]) * 1024)
+    sw = copy_bytes / (sum(cw[copy]) / len(cw[copy]) * 1024)
+    out = {"workload": workload, "fetch_scale": sf, "write_scale": sw,
+           "note": "bytes per launch = counter KiB * 1024 * scale; scale = 2 GiB / counter reading for a plain "
+                   "2 GiB device copy", "commands": commands, "kernels": {}}
+    for k in sorted(f):
+        if not k.startswith("k_"):
+            continue
+        fb = sum(f[k]) / len(f[k]) * 1024 * sf
+        wb = (sum(w[k]) / len(w[k]) * 1024 * sw) if k in w else 0.0
+        out["kernels"][k] = {"launches": len(f[k]), "fetch_bytes": fb, "write_bytes": wb}
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -141,7 +198,22 @@ def main():
     ap.add_argument("--residual-target", type=float, default=1e-6)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 2 GiB (5 times) to calibrate PMC byte counters")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("STATS_CSV", "OUT_MD"),
+                    help="write a markdown summary of a rocprofv3 kernel-stats CSV (no GPU work)")
+    ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV",
+                                                         "OUT_JSON"),
+                    help="write per-kernel HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs "
+                         "(no GPU work)")
+    ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
     args = ap.parse_args()
+    if args.summarize_trace:
+        md = summarize_trace(args.summarize_trace[0], args.workload, "; ".join(args.command))
+        open(args.summarize_trace[1], "w").write(md)
+        return
+    if args.summarize_pmc:
+        out = summarize_pmc(*args.summarize_pmc[:4], args.workload, args.command)
+        json.dump(out, open(args.summarize_pmc[4], "w"), indent=1)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the kinetic step is built for one MI355X in this version "
               "(multi-GPU velocity sharding is a design, not implemented); run with --gpus 1", file=sys.stderr)
@@ -217,7 +289,8 @@ def main():
                        "inner_sweeps": inner, "block_passes": passes, "sweep_blocks": setup.n_blocks,
                        "chunk_width": chunk, "delta_rp": 10.0},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic(args.workload, dom, precision),
                          "avg_launch_ms": round(1e3 * avg_launch_s, 4),
                          "bytes_per_launch": int(bytes_per_launch)},
             "step_roofline": {"bytes_per_update": b_alg, "achieved": round(value * b_alg, 2),

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -371,24 +372,61 @@ struct VelRange {
   int end[kMaxDirs];
 };
 
-// One cell of one sweep (forward or backward) for one direction, in level order: its faces in CSR
-// order with the class each has in that sweep, so the kernel needs no order / position / boundary
-// lookups. code = neighbour << 2 | class; class 0 skip (periodic), 1 contributes a+ only (boundary,
-// or an in-block neighbour on the downstream side), 2 in-block upstream neighbour, 3 out-of-block
-// neighbour. nf < 0: more than kGather faces, use the CSR path.
+// One cell of one sweep (forward or backward) for one direction: its faces in CSR order with the
+// class each has in that sweep, so the kernel needs no order / position / boundary lookups.
+// code = neighbour << 2 | class; class 0 skip (periodic), 1 contributes a+ only (boundary, or an
+// in-block neighbour on the downstream side), 2 in-block upstream neighbour, 3 out-of-block
+// neighbour. nf = -1: more than kGather faces, use the CSR path.
+// diag is the cell's V/tau + V/dt, copied in by k_record_diag whenever the cell parameters change.
+//
+// The records of a (direction, block) are stored level by level; a level wider than one LDS stage
+// is cut into several stages (its cells are independent, so the cut only adds a barrier).
 template <class F>
 struct SweepRec {
   int cell, nf;
   int code[kGather];
+  F diag;
   F sx[kGather], sy[kGather], sz[kGather];
 };
+template <class F>
+constexpr int kRecWords = static_cast<int>(sizeof(SweepRec<F>) / 4);
+static_assert(sizeof(SweepRec<double>) % 8 == 0 && kRecWords<double> <= 64, "record must fit one wave load");
+static_assert(sizeof(SweepRec<float>) % 4 == 0 && kRecWords<float> <= 64, "record must fit one wave load");
 
-struct LevelTables {
-  const int* cells;     // cells in (direction, block, level) order
-  const int* ptr;       // level boundaries; each (direction, block) group ends with a sentinel
-  const int* gb_begin;  // first ptr index of each (direction, block) group, plus the end
-  const int* vel_list;  // storage positions grouped by direction
-};
+// LDS staging of the records: a stage holds up to kStageRegs * threads words; two stage buffers
+// alternate (one is read while the next stage loads into the other).
+constexpr int kMaxSweepThreads = 512;
+constexpr int kStageRegs = 6;
+constexpr int kStageWords = kStageRegs * kMaxSweepThreads;
+template <class F>
+constexpr int stage_capacity(int threads) {
+  return kStageRegs * threads / kRecWords<F>;
+}
+
+template <class F>
+__global__ void k_record_diag(const CellP<F>* __restrict__ cells, SweepRec<F>* __restrict__ recs, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) recs[i].diag = cells[recs[i].cell].diag;
+}
+
+// Row access with a wave-uniform row pointer (scalar registers) plus the lane's fixed byte offset,
+// so no per-lane 64-bit address arithmetic is spent per neighbour.
+template <class F>
+__device__ __forceinline__ F row_load(const F* row, unsigned byte_off) {
+  return *reinterpret_cast<const F*>(reinterpret_cast<const char*>(row) + byte_off);
+}
+template <class F>
+__device__ __forceinline__ F& row_ref(F* row, unsigned byte_off) {
+  return *reinterpret_cast<F*>(reinterpret_cast<char*>(row) + byte_off);
+}
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Values loaded once before a loop: an empty register-constraint statement marks them as defined
+// there, so the compiler does not re-wait on their loads (and on every memory access issued after
+// them, stores included) inside the loop.
+template <class F>
+__device__ __forceinline__ void settle(F& x, F& y, F& z) {
+  asm volatile("" : "+v"(x), "+v"(y), "+v"(z));
+}
 
 // one cell of the forward sweep (sweep_forward_cell, kinetic_kernels.hpp:290-324); the upwind
 // neighbour values are gathered before the in-order accumulation
@@ -518,146 +556,209 @@ __device__ __forceinline__ void backward_cell(const View<F>& v, int cell, int t,
   if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
 }
 
-// forward cell from a sweep record: same operations and order as forward_cell
+// One cell from its sweep record in LDS: same results as forward_cell. The record's integers are
+// made wave-uniform (scalar registers), its areas are LDS operands. A neighbour row is loaded for
+// the whole wave when any lane is upwind of that face (one scalar branch, no per-lane masks), and
+// the accumulation selects per lane: adding +0 for a face a lane skips leaves every partial sum
+// unchanged bit for bit (the sums start at +0 and never become -0), so the sums equal the
+// sequential ones. toff = the lane velocity's byte offset in a chunk row.
 template <class F>
-__device__ __forceinline__ void forward_rec(const View<F>& v, const SweepRec<F>& r, int t, F x, F y, F z, int stride,
-                                            const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
-  const i64 wi = static_cast<i64>(r.cell) * stride + t;
-  const F Ri = rres[wi];
-  const F diag = v.cells[r.cell].diag;
+__device__ __forceinline__ void forward_rec(const View<F>& v, const SweepRec<F>& r, int nf, F x, F y, F z,
+                                            unsigned toff, int stride, const F* __restrict__ rres, F* cur,
+                                            const F* __restrict__ prev) {
+  const i64 row = static_cast<i64>(uniform(r.cell)) * stride;
+  const F Ri = row_load(rres + row, toff);
   F ak[kGather], nb[kGather];
+  int cls[kGather];
 #pragma unroll
-  for (int k = 0; k < kGather; ++k)
-    if (k < r.nf) {
-      const int cls = r.code[k] & 3;
+  for (int k = 0; k < kGather; ++k) {
+    cls[k] = 0;
+    ak[k] = F(0);
+    nb[k] = F(0);
+    if (k < nf) {
+      const int code = uniform(r.code[k]);
+      cls[k] = code & 3;
       ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
-      nb[k] = F(0);
-      if (cls >= 2 && !(ak[k] > F(0))) {
-        const i64 wj = static_cast<i64>(r.code[k] >> 2) * stride + t;
-        if (cls == 2)
-          nb[k] = cur[wj];
-        else if (prev != nullptr)
-          nb[k] = prev[wj];
-      }
+      const F* base = cls[k] == 2 ? cur : (cls[k] == 3 ? prev : nullptr);
+      if (base != nullptr && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
+        nb[k] = row_load(base + static_cast<i64>(code >> 2) * stride, toff);
     }
+  }
   F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
 #pragma unroll
-  for (int k = 0; k < kGather; ++k)
-    if (k < r.nf) {
-      const int cls = r.code[k] & 3;
-      if (cls == 0) continue;
-      const F a = ak[k];
-      if (a > F(0)) {
-        ap += a;
-      } else if (cls == 3) {
-        if (prev != nullptr)
-          Lout += a * nb[k];
-        else
-          am_out -= a;
-      } else if (cls == 2) {
-        L += a * nb[k];
-      }
+  for (int k = 0; k < kGather; ++k) {
+    if (cls[k] == 0) continue;
+    const F a = ak[k];
+    const bool down = a > F(0);
+    ap += down ? a : F(0);
+    if (cls[k] == 3) {
+      if (prev != nullptr)
+        Lout += down ? F(0) : a * nb[k];
+      else
+        am_out -= down ? F(0) : a;
+    } else if (cls[k] == 2) {
+      L += down ? F(0) : a * nb[k];
     }
-  const F D = diag + v.diag_scale * ap + v.interface_damping * am_out;
-  cur[wi] = (-Ri - Lout - L) / D;
+  }
+  const F D = r.diag + v.diag_scale * ap + v.interface_damping * am_out;
+  row_ref(cur + row, toff) = (-Ri - Lout - L) / D;
 }
 
-// backward cell from a sweep record: same operations and order as backward_cell
+// One cell from its sweep record in LDS: same results as backward_cell (same scheme as
+// forward_rec). poff = the lane velocity's byte offset in an f row.
 template <class F>
-__device__ __forceinline__ void backward_rec(const View<F>& v, const SweepRec<F>& r, int t, int p, F x, F y, F z,
-                                             int stride, F* cur, int has_prev, int update_f) {
-  const i64 wi = static_cast<i64>(r.cell) * stride + t;
-  const F own = cur[wi];
-  const F diag = v.cells[r.cell].diag;
+__device__ __forceinline__ void backward_rec(const View<F>& v, const SweepRec<F>& r, int nf, F x, F y, F z,
+                                             unsigned toff, unsigned poff, int stride, F* cur, int has_prev,
+                                             int update_f) {
+  const int cell = uniform(r.cell);
+  const i64 row = static_cast<i64>(cell) * stride;
+  const F own = row_load(cur + row, toff);
   F ak[kGather], nb[kGather];
+  int cls[kGather];
 #pragma unroll
-  for (int k = 0; k < kGather; ++k)
-    if (k < r.nf) {
-      const int cls = r.code[k] & 3;
+  for (int k = 0; k < kGather; ++k) {
+    cls[k] = 0;
+    ak[k] = F(0);
+    nb[k] = F(0);
+    if (k < nf) {
+      const int code = uniform(r.code[k]);
+      cls[k] = code & 3;
       ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
-      nb[k] = F(0);
-      if (cls == 2 && !(ak[k] > F(0))) nb[k] = cur[static_cast<i64>(r.code[k] >> 2) * stride + t];
+      if (cls[k] == 2 && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
+        nb[k] = row_load(cur + static_cast<i64>(code >> 2) * stride, toff);
     }
+  }
   F ap = F(0), am_out = F(0), U = F(0);
 #pragma unroll
-  for (int k = 0; k < kGather; ++k)
-    if (k < r.nf) {
-      const int cls = r.code[k] & 3;
-      if (cls == 0) continue;
-      const F a = ak[k];
-      if (a > F(0)) {
-        ap += a;
-      } else if (cls == 3) {
-        if (!has_prev) am_out -= a;
-      } else if (cls == 2) {
-        U += a * nb[k];
-      }
+  for (int k = 0; k < kGather; ++k) {
+    if (cls[k] == 0) continue;
+    const F a = ak[k];
+    const bool down = a > F(0);
+    ap += down ? a : F(0);
+    if (cls[k] == 3) {
+      if (!has_prev) am_out -= down ? F(0) : a;
+    } else if (cls[k] == 2) {
+      U += down ? F(0) : a * nb[k];
     }
-  const F D = diag + v.diag_scale * ap + v.interface_damping * am_out;
+  }
+  const F D = r.diag + v.diag_scale * ap + v.interface_damping * am_out;
   const F df = own - U / D;
-  cur[wi] = df;
-  if (update_f) v.f[static_cast<i64>(r.cell) * v.nv + p] += df;
+  row_ref(cur + row, toff) = df;
+  if (update_f) row_ref(v.f + static_cast<i64>(cell) * v.nv, poff) += df;
 }
 
+// Workgroup -> (direction, block, velocity tile). The (direction, block) pairs are dealt to the 8
+// XCDs (pair P on XCD P mod 8) and all velocity tiles of a pair run on that XCD, so the pair's
+// records are fetched into one L2 and shared by its tiles. Grid: 8 * ceil(n_pairs / 8) * tiles.
+__device__ __forceinline__ bool sweep_tile(int n_pairs, int tiles, int n_blocks, int& g, int& b, int& tile) {
+  const int x = static_cast<int>(blockIdx.x % 8);
+  const int k = static_cast<int>(blockIdx.x / 8);
+  const int pair = x + 8 * (k / tiles);
+  if (pair >= n_pairs) return false;
+  tile = k % tiles;
+  g = pair / n_blocks;
+  b = pair % n_blocks;
+  return true;
+}
+
+struct SweepArgs {
+  int k0, stride, n_blocks, n_pairs, tiles;
+  const int* block_off;    // block cell ranges
+  const int* vel_list;     // storage positions grouped by direction
+  const int* stage_off;    // first record of each stage, plus the end
+  const int* group_stage;  // first stage of each (direction, block), plus the end
+  VelRange vr;
+};
+
+// Stage s's records (contiguous in global memory) -> an LDS stage buffer, with direct
+// global-to-LDS loads (each wave moves 64 words per instruction; no registers involved). The
+// loads complete under the vector memory counter, which the stage-end barrier drains.
+#define GSK_STAGE_ISSUE(recs, stage_off, s, buf)                                                     \
+  do {                                                                                              \
+    const int* src_ = reinterpret_cast<const int*>(recs) + static_cast<i64>((stage_off)[s]) * kRecWords<F>; \
+    const int nw_ = ((stage_off)[(s) + 1] - (stage_off)[s]) * kRecWords<F>;                          \
+    for (int base_ = wave * 64; base_ < nw_; base_ += nwaves * 64)                                   \
+      if (base_ + lane < nw_) __builtin_amdgcn_global_load_lds(src_ + base_ + lane, (buf) + base_, 4, 0, 0); \
+  } while (0)
+
+// The workgroup walks the stages of its (direction, block); while it works on one stage from LDS,
+// the next stage's records are in flight to registers and go to the other LDS buffer before the
+// stage-end barrier. Records come from LDS (not through the vector memory counter), so a cell's
+// only memory round trip is its neighbour gather, which overlaps the previous cell's store. The
+// waves take every nwaves-th cell of a stage; lanes = velocities of one direction.
 template <class F>
-__global__ void k_sweep_forward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
-                                LevelTables lt, const SweepRec<F>* __restrict__ recs, VelRange vr,
+__global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_forward(View<F> v, SweepArgs a, const SweepRec<F>* __restrict__ recs,
                                 const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
-  const int g = blockIdx.y, b = blockIdx.z;
-  const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
-  if (vb >= vr.end[g]) return;
+  int g, b, tile;
+  if (!sweep_tile(a.n_pairs, a.tiles, a.n_blocks, g, b, tile)) return;
+  const int vb = a.vr.beg[g] + tile * 64;
+  if (vb >= a.vr.end[g]) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int nwaves = blockDim.x >> 6;
-  const bool valid = vb + lane < vr.end[g];
-  const int p = lt.vel_list[valid ? vb + lane : vb];
-  const int t = p - k0;
-  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
-  const int c0 = block_off[b], c1 = block_off[b + 1];
+  const bool valid = vb + lane < a.vr.end[g];
+  const int p = a.vel_list[valid ? vb + lane : vb];
+  const int t = p - a.k0;
+  const unsigned toff = static_cast<unsigned>(t) * sizeof(F), poff = static_cast<unsigned>(p) * sizeof(F);
+  F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  settle(x, y, z);
+  const int c0 = a.block_off[b], c1 = a.block_off[b + 1];
   const int* pos = v.n_sweep_dirs > 0 ? v.sweep_pos + static_cast<i64>(g) * v.n_cells : nullptr;
-  const int l0 = lt.gb_begin[g * n_blocks + b], l1 = lt.gb_begin[g * n_blocks + b + 1] - 1;
-  for (int l = l0; l < l1; ++l) {
-    const int i1 = lt.ptr[l + 1];
-    for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
-      const int cell = lt.cells[i];
+  __shared__ __align__(16) int lds[2][kStageWords];
+  const int s0 = a.group_stage[g * a.n_blocks + b], s1 = a.group_stage[g * a.n_blocks + b + 1];
+  if (s0 < s1) GSK_STAGE_ISSUE(recs, a.stage_off, s0, lds[0]);
+  __syncthreads();
+  for (int s = s0; s < s1; ++s) {
+    const int* buf = lds[(s - s0) & 1];
+    if (s + 1 < s1) GSK_STAGE_ISSUE(recs, a.stage_off, s + 1, lds[(s + 1 - s0) & 1]);
+    const int n = a.stage_off[s + 1] - a.stage_off[s];
+    for (int i = wave; i < n; i += nwaves) {
+      const SweepRec<F>& r = reinterpret_cast<const SweepRec<F>*>(buf)[i];
+      const int nf = uniform(r.nf);
       if (!valid) continue;
-      const SweepRec<F>& r = recs[i];
-      if (r.nf >= 0)
-        forward_rec(v, r, t, x, y, z, stride, rres, cur, prev);
+      if (nf >= 0)
+        forward_rec(v, r, nf, x, y, z, toff, a.stride, rres, cur, prev);
       else
-        forward_cell(v, cell, t, c0, c1, pos, x, y, z, stride, rres, cur, prev);
+        forward_cell(v, uniform(r.cell), t, c0, c1, pos, x, y, z, a.stride, rres, cur, prev);
     }
     __syncthreads();
   }
 }
 
 template <class F>
-__global__ void k_sweep_backward(View<F> v, int k0, int stride, int n_blocks, const int* __restrict__ block_off,
-                                 LevelTables lt, const SweepRec<F>* __restrict__ recs, VelRange vr, F* cur,
-                                 int has_prev, int update_f) {
-  const int g = blockIdx.y, b = blockIdx.z;
-  const int vb = vr.beg[g] + static_cast<int>(blockIdx.x) * 64;
-  if (vb >= vr.end[g]) return;
+__global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_backward(View<F> v, SweepArgs a, const SweepRec<F>* __restrict__ recs, F* cur, int has_prev,
+                                 int update_f) {
+  int g, b, tile;
+  if (!sweep_tile(a.n_pairs, a.tiles, a.n_blocks, g, b, tile)) return;
+  const int vb = a.vr.beg[g] + tile * 64;
+  if (vb >= a.vr.end[g]) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int nwaves = blockDim.x >> 6;
-  const bool valid = vb + lane < vr.end[g];
-  const int p = lt.vel_list[valid ? vb + lane : vb];
-  const int t = p - k0;
-  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
-  const int c0 = block_off[b], c1 = block_off[b + 1];
+  const bool valid = vb + lane < a.vr.end[g];
+  const int p = a.vel_list[valid ? vb + lane : vb];
+  const int t = p - a.k0;
+  const unsigned toff = static_cast<unsigned>(t) * sizeof(F), poff = static_cast<unsigned>(p) * sizeof(F);
+  F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  settle(x, y, z);
+  const int c0 = a.block_off[b], c1 = a.block_off[b + 1];
   const int* pos = v.n_sweep_dirs > 0 ? v.sweep_pos + static_cast<i64>(g) * v.n_cells : nullptr;
-  const int l0 = lt.gb_begin[g * n_blocks + b], l1 = lt.gb_begin[g * n_blocks + b + 1] - 1;
-  for (int l = l0; l < l1; ++l) {
-    const int i1 = lt.ptr[l + 1];
-    for (int i = lt.ptr[l] + wave; i < i1; i += nwaves) {
-      const int cell = lt.cells[i];
+  __shared__ __align__(16) int lds[2][kStageWords];
+  const int s0 = a.group_stage[g * a.n_blocks + b], s1 = a.group_stage[g * a.n_blocks + b + 1];
+  if (s0 < s1) GSK_STAGE_ISSUE(recs, a.stage_off, s0, lds[0]);
+  __syncthreads();
+  for (int s = s0; s < s1; ++s) {
+    const int* buf = lds[(s - s0) & 1];
+    if (s + 1 < s1) GSK_STAGE_ISSUE(recs, a.stage_off, s + 1, lds[(s + 1 - s0) & 1]);
+    const int n = a.stage_off[s + 1] - a.stage_off[s];
+    for (int i = wave; i < n; i += nwaves) {
+      const SweepRec<F>& r = reinterpret_cast<const SweepRec<F>*>(buf)[i];
+      const int nf = uniform(r.nf);
       if (!valid) continue;
-      const SweepRec<F>& r = recs[i];
-      if (r.nf >= 0)
-        backward_rec(v, r, t, p, x, y, z, stride, cur, has_prev, update_f);
+      if (nf >= 0)
+        backward_rec(v, r, nf, x, y, z, toff, poff, a.stride, cur, has_prev, update_f);
       else
-        backward_cell(v, cell, t, p, c0, c1, pos, x, y, z, stride, cur, has_prev, update_f);
+        backward_cell(v, uniform(r.cell), t, p, c0, c1, pos, x, y, z, a.stride, cur, has_prev, update_f);
     }
     __syncthreads();
   }
@@ -1039,6 +1140,10 @@ class HipBackend final : public Backend {
     begin(kOther);
     dev::k_cell_params<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), moments_.p, volume_.p, vdt_.p, mu0_, omega_, Pr_,
                                                                      cells_.p, flags_.p);
+    for (DBuf<dev::SweepRec<F>>* recs : {&fw_recs_, &bw_recs_}) {
+      const int nr = static_cast<int>(recs->n);
+      if (nr > 0) dev::k_record_diag<F><<<(nr + 255) / 256, 256, 0, stream_>>>(cells_.p, recs->p, nr);
+    }
     HIPCHK(hipGetLastError());
     end();
     deq_flags_.resize(static_cast<std::size_t>(nc_));
@@ -1127,22 +1232,25 @@ class HipBackend final : public Backend {
         vr.end[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0 + ch.kc) - vel_list_host_.data());
         max_tiles = std::max(max_tiles, (vr.end[g] - vr.beg[g] + 63) / 64);
       }
-      const dim3 sgrid(static_cast<unsigned>(std::max(max_tiles, 1)), static_cast<unsigned>(n_dirs_eff_),
-                       static_cast<unsigned>(n_blocks_));
-      const dev::LevelTables fw{fw_cells_.p, fw_ptr_.p, fw_gb_.p, vel_list_.p};
-      const dev::LevelTables bw{bw_cells_.p, bw_ptr_.p, bw_gb_.p, vel_list_.p};
+      const int tiles = std::max(max_tiles, 1);
+      const int n_pairs = n_dirs_eff_ * n_blocks_;
+      const long long sgrid_ll = 8LL * ((n_pairs + 7) / 8) * tiles;
+      GSK_CHECK(sgrid_ll < (1LL << 31), "sweep grid too large");
+      const unsigned sgrid = static_cast<unsigned>(sgrid_ll);
+      const dev::SweepArgs fw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, fw_stage_.p,
+                              fw_group_.p, vr};
+      const dev::SweepArgs bw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, bw_stage_.p,
+                              bw_group_.p, vr};
       F* cur = bufA_.p;
       F* prev = nullptr;
       for (int pass = 0; pass < block_passes_; ++pass) {
+        const int has_prev = prev != nullptr ? 1 : 0, update_f = pass + 1 == block_passes_ ? 1 : 0;
         begin(kForward);
-        dev::k_sweep_forward<F><<<sgrid, 64 * waves_fw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, fw,
-                                                                        fw_recs_.p, vr, rres_.p, cur, prev);
+        dev::k_sweep_forward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
         HIPCHK(hipGetLastError());
         end();
         begin(kBackward);
-        dev::k_sweep_backward<F><<<sgrid, 64 * waves_bw_, 0, stream_>>>(v, ch.k0, ch.stride, n_blocks_, block_off_.p, bw,
-                                                                         bw_recs_.p, vr, cur, prev != nullptr ? 1 : 0,
-                                                                         pass + 1 == block_passes_ ? 1 : 0);
+        dev::k_sweep_backward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, has_prev, update_f);
         HIPCHK(hipGetLastError());
         end();
         prev = cur;
@@ -1299,14 +1407,40 @@ class HipBackend final : public Backend {
     }
     fw_gb.push_back(static_cast<int>(fw_ptr.size()));
     bw_gb.push_back(static_cast<int>(bw_ptr.size()));
-    fw_recs_.upload(fw_recs.data(), fw_recs.size(), stream_);
-    bw_recs_.upload(bw_recs.data(), bw_recs.size(), stream_);
-    fw_cells_.upload(fw_cells.data(), fw_cells.size(), stream_);
-    bw_cells_.upload(bw_cells.data(), bw_cells.size(), stream_);
-    fw_ptr_.upload(fw_ptr.data(), fw_ptr.size(), stream_);
-    bw_ptr_.upload(bw_ptr.data(), bw_ptr.size(), stream_);
-    fw_gb_.upload(fw_gb.data(), fw_gb.size(), stream_);
-    bw_gb_.upload(bw_gb.data(), bw_gb.size(), stream_);
+    // waves per workgroup ~ mean level width (cells a level offers to run side by side)
+    const double width = total_levels > 0 ? static_cast<double>(n_dirs_eff_) * nc / static_cast<double>(total_levels) : 1.0;
+    int w = 1;
+    while (w < 8 && 2 * w <= width) w *= 2;
+    // tuning knob: GSK_SWEEP_WAVES overrides the waves per sweep workgroup (1..8)
+    if (const char* env = std::getenv("GSK_SWEEP_WAVES")) {
+      const int e = std::atoi(env);
+      if (e >= 1 && e <= dev::kMaxSweepThreads / 64) w = e;
+    }
+    waves_ = w;
+    mean_level_width_ = width;
+    // LDS stages: each level of each (direction, block), cut into pieces of at most cap records
+    const int cap = dev::stage_capacity<F>(64 * w);
+    GSK_CHECK(cap >= 1, "sweep stage holds no record");
+    for (int dirn = 0; dirn < 2; ++dirn) {
+      const std::vector<int>& ptr = dirn == 0 ? fw_ptr : bw_ptr;
+      const std::vector<int>& gb = dirn == 0 ? fw_gb : bw_gb;
+      const std::vector<dev::SweepRec<F>>& recs = dirn == 0 ? fw_recs : bw_recs;
+      std::vector<int> stage_off, group_stage;
+      const int n_groups = static_cast<int>(gb.size()) - 1;
+      GSK_CHECK(n_groups == n_dirs_eff_ * P, "sweep level tables do not cover every (direction, block)");
+      for (int q = 0; q < n_groups; ++q) {
+        group_stage.push_back(static_cast<int>(stage_off.size()));
+        const int l0 = gb[static_cast<std::size_t>(q)], l1 = gb[static_cast<std::size_t>(q) + 1] - 1;
+        for (int l = l0; l < l1; ++l)
+          for (int i = ptr[static_cast<std::size_t>(l)]; i < ptr[static_cast<std::size_t>(l) + 1]; i += cap)
+            stage_off.push_back(i);
+      }
+      group_stage.push_back(static_cast<int>(stage_off.size()));
+      stage_off.push_back(static_cast<int>(recs.size()));
+      (dirn == 0 ? fw_recs_ : bw_recs_).upload(recs.data(), recs.size(), stream_);
+      (dirn == 0 ? fw_stage_ : bw_stage_).upload(stage_off.data(), stage_off.size(), stream_);
+      (dirn == 0 ? fw_group_ : bw_group_).upload(group_stage.data(), group_stage.size(), stream_);
+    }
     // velocities grouped by direction (ascending storage position inside a direction)
     vel_list_host_.clear();
     dir_off_.assign(1, 0);
@@ -1317,12 +1451,6 @@ class HipBackend final : public Backend {
     }
     GSK_CHECK(static_cast<int>(vel_list_host_.size()) == nv_, "sweep direction table does not cover the velocities");
     vel_list_.upload(vel_list_host_.data(), vel_list_host_.size(), stream_);
-    // waves per workgroup ~ mean level width (cells a level offers to run side by side)
-    const double width = total_levels > 0 ? static_cast<double>(n_dirs_eff_) * nc / static_cast<double>(total_levels) : 1.0;
-    int w = 1;
-    while (w < 8 && 2 * w <= width) w *= 2;
-    waves_fw_ = waves_bw_ = w;
-    mean_level_width_ = width;
   }
 
   struct EventPair {
@@ -1458,9 +1586,9 @@ class HipBackend final : public Backend {
   DBuf<double> moments_, shift_, residual_, bsums_, partial_;
   DBuf<int> flags_;
   DBuf<F> gx_, gy_, gz_, rres_, bufA_, bufB_;
-  int n_dirs_eff_ = 1, waves_fw_ = 1, waves_bw_ = 1;
+  int n_dirs_eff_ = 1, waves_ = 1;
   double mean_level_width_ = 1.0;
-  DBuf<int> fw_cells_, bw_cells_, fw_ptr_, bw_ptr_, fw_gb_, bw_gb_, vel_list_;
+  DBuf<int> fw_stage_, bw_stage_, fw_group_, bw_group_, vel_list_;
   DBuf<dev::SweepRec<F>> fw_recs_, bw_recs_;
   DBuf<int> subset_;
   std::vector<int> vel_list_host_, dir_off_;
