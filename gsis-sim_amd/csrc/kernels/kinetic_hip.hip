@@ -69,9 +69,14 @@ __device__ __forceinline__ F block_extreme(F v, F* lds) {
   return r;
 }
 
-// Discrete conservative Maxwellian of one cell, all threads of the block cooperating on the 20
-// sums of each Newton step; every thread ends with the same result (discrete_maxwell_seq,
-// kinetic_kernels.hpp:499-540). Returns iterations, 0 when resolved, -1 on failure.
+// Discrete conservative Maxwellian of one cell, all threads of the block cooperating on the sums
+// of each Newton step; every thread ends with the same result (discrete_maxwell_seq,
+// kinetic_kernels.hpp:499-540). A Newton step needs the 15 conservation / Jacobian sums; the 5
+// Shakhov-polynomial sums enter only the correction gamma of the accepted iterate, so they are
+// summed in one extra pass for that iterate (each sum is reduced on its own, in the same order, so
+// the values equal those of a single 20-sum pass). Returns iterations, 0 when resolved, -1 on
+// failure.
+constexpr int kDeqNewtonSums = 15;
 template <class F>
 __device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, double uy, double uz, double T,
                                       const Target<F>* poly, Maxw<F>& out, double (&gamma)[5], double* lds) {
@@ -85,27 +90,75 @@ __device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, d
   for (int it = 0; it < v.deq.max_iterations; ++it) {
     const Maxw<F> m = deq_params<F>(beta, ux, uy, uz);
     double sums[kDeqSums];
+#pragma unroll
     for (int k = 0; k < kDeqSums; ++k) sums[k] = 0.0;
     for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
       const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
       const F cx = x - fux, cy = y - fuy, cz = z - fuz;
-      const F e = maxw_eval(m, x, y, z);
-      const F pm = poly != nullptr ? shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz) : F(0);
       F t[kDeqSums];
-      deq_terms<F>(e, pm, cx, cy, cz, t);
-      for (int k = 0; k < kDeqSums; ++k) sums[k] += static_cast<double>(t[k]);
+      deq_terms<F>(maxw_eval(m, x, y, z), F(0), cx, cy, cz, t);
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] += static_cast<double>(t[k]);
     }
-    block_sum<kDeqSums>(sums, lds);
-    double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
-    double rel = 0.0;
-    const int r = deq_newton_step(sums, v.weight_d, rho, T, v.deq.tolerance, b, rel);
-    if (deq_accept(r, rel, rel_prev, it + 1 == v.deq.max_iterations)) {
+    {
+      double s15[kDeqNewtonSums];
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) s15[k] = sums[k];
+      block_sum<kDeqNewtonSums>(s15, lds);
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] = s15[k];
+    }
+    // The 5x5 Newton step runs in one thread, whose pivoting arrays live in private memory; the
+    // block reads its result from LDS.
+    if (threadIdx.x == 0) {
+      double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
+      double rel = 0.0;
+      const int r = deq_newton_step(sums, v.weight_d, rho, T, v.deq.tolerance, b, rel);
+      const bool accept = deq_accept(r, rel, rel_prev, it + 1 == v.deq.max_iterations);
+      for (int k = 0; k < 5; ++k) lds[k] = b[k];
+      lds[5] = rel;
+      lds[6] = static_cast<double>(r);
+      lds[7] = accept ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const double rel = lds[5];
+    const int r = static_cast<int>(lds[6]);
+    const bool accept = lds[7] != 0.0;
+    double b[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) b[k] = lds[k];
+    __syncthreads();
+    if (accept) {
       out = m;
-      if (poly != nullptr && !deq_shakhov_correction(sums, v.weight_d, gamma))
-        for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+      if (poly != nullptr) {
+        double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+          const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+          const F cx = x - fux, cy = y - fuy, cz = z - fuz;
+          const F pm = shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz);
+          F t[kDeqSums];
+          deq_terms<F>(maxw_eval(m, x, y, z), pm, cx, cy, cz, t);
+#pragma unroll
+          for (int k = 0; k < 5; ++k) s5[k] += static_cast<double>(t[kDeqNewtonSums + k]);
+        }
+        block_sum<5>(s5, lds);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) sums[kDeqNewtonSums + k] = s5[k];
+        if (threadIdx.x == 0) {
+          double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+          if (!deq_shakhov_correction(sums, v.weight_d, g))
+            for (int k = 0; k < 5; ++k) g[k] = 0.0;
+          for (int k = 0; k < 5; ++k) lds[k] = g[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gamma[k] = lds[k];
+        __syncthreads();
+      }
       return it + 1;
     }
     if (r < 0) break;
+#pragma unroll
     for (int k = 0; k < 5; ++k) beta[k] = b[k];
     rel_prev = rel;
   }
@@ -235,53 +288,73 @@ __global__ void k_maxwell_reflection(View<F> v, F* __restrict__ out) {
   out[v.mw_off[b] + p] = (x * bf.nx + y * bf.ny + z * bf.nz) < F(0) ? reflected_value<Reconstruct>(v, bf, p, x, y, z) : F(0);
 }
 
-// 20 half-space sums per boundary face in FP64 (k_boundary_sums). faces: optional subset list.
+// 20 half-space sums per boundary face in FP64 (k_boundary_sums): flux and moment sums (5 each)
+// over the outgoing velocities (slots 0..4, 10..14) and over the incoming ones (5..9, 15..19).
+// faces: optional subset list. Block z = 0 sums the outgoing side, z = 1 the incoming side (each
+// block holds 10 accumulators); the velocities of a face are split over gridDim.y blocks (block y
+// takes every gridDim.y-th run of blockDim velocities). k_boundary_reduce adds the partials in order.
+constexpr int kSideSums = 10;
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_boundary_sums(View<F> v, const int* __restrict__ faces, int n_faces,
-                                                          double* __restrict__ out) {
-  __shared__ double lds[kWaves * kBoundarySums];
+                                                          double* __restrict__ part) {
+  __shared__ double lds[kWaves * kSideSums];
+  const int nchunk = static_cast<int>(gridDim.y), chunk = static_cast<int>(blockIdx.y);
+  const bool out_side = blockIdx.z == 0;
   for (int i = blockIdx.x; i < n_faces; i += gridDim.x) {
     const int b = faces ? faces[i] : i;
     const BFace<F> bf = v.bf[b];
-    double acc[kBoundarySums];
-    for (int k = 0; k < kBoundarySums; ++k) acc[k] = 0.0;
-    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+    double acc[kSideSums];
+#pragma unroll
+    for (int k = 0; k < kSideSums; ++k) acc[k] = 0.0;
+    for (int p = chunk * blockDim.x + threadIdx.x; p < v.nv; p += nchunk * blockDim.x) {
       const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
       const F an = x * bf.nx + y * bf.ny + z * bf.nz;
+      const bool outgoing = an >= F(0) || bf.kind == kBcPeriodic;
+      if (outgoing != out_side) continue;
       F fb;
-      int base;
-      if (an >= F(0) || bf.kind == kBcPeriodic) {
+      if (outgoing) {
         fb = reconstruct_at_boundary(v, bf, p, x, y, z);
-        base = 0;
+      } else if (bf.kind == kBcSymmetry) {
+        const int pm = p ^ bf.mirror_xor;
+        fb = reconstruct_at_boundary(v, bf, pm, v.xi_x[pm], v.xi_y[pm], v.xi_z[pm]);
+      } else if (bf.kind == kBcMaxwell) {
+        fb = maxwell_inflow(v, b, p, x, y, z);
       } else {
-        if (bf.kind == kBcSymmetry) {
-          const int pm = p ^ bf.mirror_xor;
-          fb = reconstruct_at_boundary(v, bf, pm, v.xi_x[pm], v.xi_y[pm], v.xi_z[pm]);
-        } else if (bf.kind == kBcMaxwell) {
-          fb = maxwell_inflow(v, b, p, x, y, z);
-        } else {
-          fb = maxw_eval(v.inflow[b].m, x, y, z);
-        }
-        base = 5;
+        fb = maxw_eval(v.inflow[b].m, x, y, z);
       }
       const F wf = v.weight * fb;
       const F e = F(0.5) * (x * x + y * y + z * z);
       const F af = an * wf;
-      const double tf[5] = {static_cast<double>(af), static_cast<double>(af * x), static_cast<double>(af * y),
-                            static_cast<double>(af * z), static_cast<double>(af * e)};
-      const double tm[5] = {static_cast<double>(wf), static_cast<double>(wf * x), static_cast<double>(wf * y),
-                            static_cast<double>(wf * z), static_cast<double>(wf * e)};
-      // branch-free accumulation into the outgoing (0..4, 10..14) or incoming (5..9, 15..19) slots
-      for (int k = 0; k < 5; ++k) {
-        acc[k] += base == 0 ? tf[k] : 0.0;
-        acc[5 + k] += base == 5 ? tf[k] : 0.0;
-        acc[10 + k] += base == 0 ? tm[k] : 0.0;
-        acc[15 + k] += base == 5 ? tm[k] : 0.0;
-      }
+      acc[0] += static_cast<double>(af);
+      acc[1] += static_cast<double>(af * x);
+      acc[2] += static_cast<double>(af * y);
+      acc[3] += static_cast<double>(af * z);
+      acc[4] += static_cast<double>(af * e);
+      acc[5] += static_cast<double>(wf);
+      acc[6] += static_cast<double>(wf * x);
+      acc[7] += static_cast<double>(wf * y);
+      acc[8] += static_cast<double>(wf * z);
+      acc[9] += static_cast<double>(wf * e);
     }
-    block_sum<kBoundarySums>(acc, lds);
-    if (threadIdx.x < kBoundarySums) out[static_cast<i64>(b) * kBoundarySums + threadIdx.x] = acc[threadIdx.x];
+    block_sum<kSideSums>(acc, lds);
+    if (threadIdx.x == 0) {
+      double* o = part + ((static_cast<i64>(i) * nchunk + chunk) * 2 + blockIdx.z) * kSideSums;
+#pragma unroll
+      for (int k = 0; k < kSideSums; ++k) o[k] = acc[k];
+    }
   }
+}
+
+__global__ void k_boundary_reduce(const int* __restrict__ faces, int n_faces, int nchunk, const double* __restrict__ part,
+                                  double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_faces * kBoundarySums) return;
+  const int i = j / kBoundarySums, k = j % kBoundarySums;
+  const int b = faces ? faces[i] : i;
+  const int side = (k % 10) / 5, slot = (k / 10) * 5 + k % 5;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[((static_cast<i64>(i) * nchunk + c) * 2 + side) * kSideSums + slot];
+  out[static_cast<i64>(b) * kBoundarySums + k] = s;
 }
 
 // section mass flux through listed internal faces (k_internal_face_mass_flux)
@@ -303,8 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int*
 // XCD-aware (cell, velocity tile) mapping. Workgroups are dealt round-robin to the 8 XCDs; XCD x
 // takes the velocity tiles t = x (mod 8) of every cell, cells in order. The f / gradient rows that a
 // cell re-reads from its neighbours (close in the RCB/RCM numbering) were then touched by the same
-// XCD a few hundred workgroups earlier, with only an eighth of the velocities in between, so they
-// are still in that XCD's L2. Grid: 8 * ceil(tiles / 8) * n_cells workgroups.
+// XCD a few hundred workgroups earlier, with only an eighth of the velocities in between. (Walking
+// all cells of one tile first, with 256- or 64-velocity tiles, reads rows at the stride of a full
+// velocity row and measured slower.) Grid: 8 * ceil(tiles / 8) * n_cells workgroups.
 __device__ __forceinline__ bool xcd_tile(int n_cells, int tiles, int& cell, int& tile) {
   const int x = static_cast<int>(blockIdx.x % 8);
   const i64 k = blockIdx.x / 8;
@@ -818,7 +892,7 @@ __global__ __launch_bounds__(kBlock) void k_gsis_correction(View<F> v, const dou
       }
       replace = theta > F(0);
     } else {
-      F fmin = F(1e300 > 1e38 ? 3.0e38 : 0);
+      F fmin = static_cast<F>(3.0e38);
       bool any = false;
       for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
         const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
@@ -1041,6 +1115,8 @@ class HipBackend final : public Backend {
     HIPCHK(hipMemsetAsync(flags_.p, 0, flags_.bytes(), stream_));
     residual_.alloc(static_cast<std::size_t>(nc_));
     bsums_.alloc(static_cast<std::size_t>(nb_) * dev::kBoundarySums + 1);
+    bsum_chunks_ = std::max(1, std::min(16, (nv_ + 2047) / 2048));
+    bsum_part_.alloc(static_cast<std::size_t>(nb_) * bsum_chunks_ * 2 * dev::kSideSums + 1);
     // chunk width: requested (rounded up to the orbit size), or as wide as 6 workspace arrays fit
     const int orbit = s.orbit_size;
     long long k = s.velocity_chunk > 0 ? s.velocity_chunk : nv_;
@@ -1085,7 +1161,8 @@ class HipBackend final : public Backend {
   std::size_t memory_bytes() const override {
     return f_.bytes() + gx_.bytes() + gy_.bytes() + gz_.bytes() + rres_.bytes() + bufA_.bytes() + bufB_.bytes() +
            csr_.bytes() + moments_.bytes() + specular_.bytes() + specular_first_.bytes() + partial_.bytes() +
-           cells_.bytes() + (n_sweep_dirs_ > 0 ? sweep_order_.bytes() + sweep_pos_.bytes() : 0);
+           cells_.bytes() + fw_recs_.bytes() + bw_recs_.bytes() + bsum_part_.bytes() +
+           (n_sweep_dirs_ > 0 ? sweep_order_.bytes() + sweep_pos_.bytes() : 0);
   }
 
   void set_equilibrium(const double* states, std::size_t n) override {
@@ -1180,7 +1257,7 @@ class HipBackend final : public Backend {
       dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
       dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
     }
-    if (nb_ > 0) dev::k_boundary_sums<F><<<std::min(nb_, 65536), dev::kBlock, 0, stream_>>>(view(), nullptr, nb_, bsums_.p);
+    if (nb_ > 0) launch_boundary_sums(nullptr, nb_);
     HIPCHK(hipGetLastError());
     end();
     download(out, bsums_.p, n);
@@ -1196,12 +1273,22 @@ class HipBackend final : public Backend {
       dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
       dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
     }
-    subset_.upload(faces, n_faces, stream_);
-    dev::k_boundary_sums<F><<<static_cast<unsigned>(std::min<std::size_t>(n_faces, 65536)), dev::kBlock, 0, stream_>>>(
-        view(), subset_.p, static_cast<int>(n_faces), bsums_.p);
+    if (subset_host_.size() != n_faces || !std::equal(faces, faces + n_faces, subset_host_.begin())) {
+      subset_host_.assign(faces, faces + n_faces);
+      subset_.upload(faces, n_faces, stream_);
+    }
+    launch_boundary_sums(subset_.p, static_cast<int>(n_faces));
     HIPCHK(hipGetLastError());
     end();
     download(out, bsums_.p, n);
+  }
+
+  // partial sums over velocity chunks, then their fixed-order sum into bsums_
+  void launch_boundary_sums(const int* faces, int n_faces) {
+    const dim3 grid(static_cast<unsigned>(std::min(n_faces, 65535)), static_cast<unsigned>(bsum_chunks_), 2);
+    dev::k_boundary_sums<F><<<grid, dev::kBlock, 0, stream_>>>(view(), faces, n_faces, bsum_part_.p);
+    const int nj = n_faces * dev::kBoundarySums;
+    dev::k_boundary_reduce<<<(nj + 255) / 256, 256, 0, stream_>>>(faces, n_faces, bsum_chunks_, bsum_part_.p, bsums_.p);
   }
 
   void sweep(double* residual_cell, std::size_t n) override {
@@ -1591,6 +1678,9 @@ class HipBackend final : public Backend {
   DBuf<int> fw_stage_, bw_stage_, fw_group_, bw_group_, vel_list_;
   DBuf<dev::SweepRec<F>> fw_recs_, bw_recs_;
   DBuf<int> subset_;
+  std::vector<int> subset_host_;
+  DBuf<double> bsum_part_;
+  int bsum_chunks_ = 1;
   std::vector<int> vel_list_host_, dir_off_;
   std::vector<int> deq_flags_;
   std::vector<int> host_bface_kind_;
