@@ -36,6 +36,14 @@ class Backend {
     boundary_sums(out, n);
   }
   virtual gsk_deq_counts deq_counts() = 0;
+  // Discrete-equilibrium correction of boundary inflow states in place (the FP64 Newton of the
+  // reference's KineticSolver::correct_inflow, kinetic_solver.cpp:103-114). Returns false when the
+  // backend does not provide it; the solver then corrects the states on the host.
+  virtual bool correct_inflow(gsk_inflow_state* states, std::size_t n) {
+    (void)states;
+    (void)n;
+    return false;
+  }
 };
 
 // HIP backend on the current device (FP64 or FP32 f per view.precision_fp64)

@@ -246,9 +246,16 @@ void KineticSolver::update_pressure_inflow() {  // (:172-260)
       pending.push_back(b);
     }
   }
-  // discrete-equilibrium correction of the new inflow states: independent per face
+  // discrete-equilibrium correction of the new inflow states: independent per face, on the
+  // backend's device when it provides it, else on the host cores
+  if (!s_.discrete_eq || pending.empty()) return;
+  std::vector<gsk_inflow_state> batch(pending.size());
+  for (std::size_t i = 0; i < pending.size(); ++i) batch[i] = inflow_[static_cast<std::size_t>(pending[i])];
+  if (!backend_->correct_inflow(batch.data(), batch.size())) {
 #pragma omp parallel for schedule(dynamic, 4)
-  for (std::size_t i = 0; i < pending.size(); ++i) correct_inflow(inflow_[static_cast<std::size_t>(pending[i])]);
+    for (std::size_t i = 0; i < batch.size(); ++i) correct_inflow(batch[i]);
+  }
+  for (std::size_t i = 0; i < pending.size(); ++i) inflow_[static_cast<std::size_t>(pending[i])] = batch[i];
 }
 
 std::array<double, 10> KineticSolver::compute_wall_unit_sums(Index b) const {  // (:263-277)

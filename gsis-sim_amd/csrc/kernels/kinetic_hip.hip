@@ -76,24 +76,28 @@ __device__ __forceinline__ F block_extreme(F v, F* lds) {
 // summed in one extra pass for that iterate (each sum is reduced on its own, in the same order, so
 // the values equal those of a single 20-sum pass). Returns iterations, 0 when resolved, -1 on
 // failure.
+// F is the type of the velocity points and of the terms: the f precision for cells, double for
+// boundary inflow states (which the reference corrects in FP64 on the host).
 constexpr int kDeqNewtonSums = 15;
 template <class F>
-__device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, double uy, double uz, double T,
-                                      const Target<F>* poly, Maxw<F>& out, double (&gamma)[5], double* lds) {
+__device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F* __restrict__ ys,
+                                      const F* __restrict__ zs, double weight, const DeqGrid& grid, double tolerance,
+                                      double rho, double ux, double uy, double uz, double T, const Target<F>* poly,
+                                      Maxw<F>& out, double (&gamma)[5], double* lds) {
   for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
   out = maxw_from_state<F>(rho, ux, uy, uz, T);
-  if (deq_resolved(v.deq, ux, uy, uz, T)) return 0;
+  if (deq_resolved(grid, ux, uy, uz, T)) return 0;
   double beta[5];
   deq_initial_beta(rho, T, beta);
   const F fux = static_cast<F>(ux), fuy = static_cast<F>(uy), fuz = static_cast<F>(uz);
   double rel_prev = 1e300;
-  for (int it = 0; it < v.deq.max_iterations; ++it) {
+  for (int it = 0; it < grid.max_iterations; ++it) {
     const Maxw<F> m = deq_params<F>(beta, ux, uy, uz);
     double sums[kDeqSums];
 #pragma unroll
     for (int k = 0; k < kDeqSums; ++k) sums[k] = 0.0;
-    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
-      const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+    for (int p = threadIdx.x; p < nv; p += blockDim.x) {
+      const F x = xs[p], y = ys[p], z = zs[p];
       const F cx = x - fux, cy = y - fuy, cz = z - fuz;
       F t[kDeqSums];
       deq_terms<F>(maxw_eval(m, x, y, z), F(0), cx, cy, cz, t);
@@ -113,8 +117,8 @@ __device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, d
     if (threadIdx.x == 0) {
       double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
       double rel = 0.0;
-      const int r = deq_newton_step(sums, v.weight_d, rho, T, v.deq.tolerance, b, rel);
-      const bool accept = deq_accept(r, rel, rel_prev, it + 1 == v.deq.max_iterations);
+      const int r = deq_newton_step(sums, weight, rho, T, tolerance, b, rel);
+      const bool accept = deq_accept(r, rel, rel_prev, it + 1 == grid.max_iterations);
       for (int k = 0; k < 5; ++k) lds[k] = b[k];
       lds[5] = rel;
       lds[6] = static_cast<double>(r);
@@ -132,8 +136,8 @@ __device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, d
       out = m;
       if (poly != nullptr) {
         double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
-          const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+        for (int p = threadIdx.x; p < nv; p += blockDim.x) {
+          const F x = xs[p], y = ys[p], z = zs[p];
           const F cx = x - fux, cy = y - fuy, cz = z - fuz;
           const F pm = shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz);
           F t[kDeqSums];
@@ -146,7 +150,7 @@ __device__ int block_discrete_maxwell(const View<F>& v, double rho, double ux, d
         for (int k = 0; k < 5; ++k) sums[kDeqNewtonSums + k] = s5[k];
         if (threadIdx.x == 0) {
           double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-          if (!deq_shakhov_correction(sums, v.weight_d, g))
+          if (!deq_shakhov_correction(sums, weight, g))
             for (int k = 0; k < 5; ++k) g[k] = 0.0;
           for (int k = 0; k < 5; ++k) lds[k] = g[k];
         }
@@ -174,12 +178,42 @@ __device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, d
   if (!v.discrete_eq) return maxw_from_state<F>(rho, ux, uy, uz, T);
   Maxw<F> m;
   double gamma[5];
-  const int r = block_discrete_maxwell<F>(v, rho, ux, uy, uz, T, nullptr, m, gamma, lds);
+  const int r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.deq, v.deq.tolerance, rho, ux, uy,
+                                          uz, T, nullptr, m, gamma, lds);
   if (result) *result = r;
   return m;
 }
 
 // ---- kernels ------------------------------------------------------------------------------
+
+// Discrete-equilibrium correction of boundary inflow states in FP64, one block per state
+// {scale, rho, ux, uy, uz, T}: a state that is not resolved by the grid is replaced by the
+// Maxwellian whose discrete moments match it (KineticSolver::correct_inflow, kinetic_solver.cpp:103-114;
+// maxwell_params_to_state, kinetic_kernels.hpp:543-552). Newton tolerance 1e-10.
+__global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double* __restrict__ xd,
+                                                           const double* __restrict__ yd, const double* __restrict__ zd,
+                                                           double weight, DeqGrid grid, double* __restrict__ states,
+                                                           int n) {
+  __shared__ double lds[kWaves * kDeqSums];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    double* st = states + static_cast<i64>(i) * 6;
+    const double rho = st[1], ux = st[2], uy = st[3], uz = st[4], T = st[5];
+    if (!(rho > 0.0) || !(T > 0.0)) continue;
+    Maxw<double> m;
+    double gamma[5];
+    const int r = block_discrete_maxwell<double>(nv, xd, yd, zd, weight, grid, 1e-10, rho, ux, uy, uz, T, nullptr, m,
+                                                 gamma, lds);
+    if (r > 0 && threadIdx.x == 0) {
+      const double Tn = 1.0 / (2.0 * m.B);
+      const double tpt = 2.0 * kPi * Tn;
+      st[1] = m.A * tpt * ::sqrt(tpt);
+      st[2] = m.ux;
+      st[3] = m.uy;
+      st[4] = m.uz;
+      st[5] = Tn;
+    }
+  }
+}
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_set_equilibrium(View<F> v, const double* __restrict__ states,
@@ -256,7 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double*
     if (v.discrete_eq) {
       Maxw<F> eq;
       double gamma[5];
-      r = block_discrete_maxwell<F>(v, m[0], m[1], m[2], m[3], m[4], &cp.s, eq, gamma, lds);
+      r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.deq, v.deq.tolerance, m[0], m[1], m[2],
+                                    m[3], m[4], &cp.s, eq, gamma, lds);
       if (r > 0) {
         cp.s.eq = eq;
         cp.s.g0 = static_cast<F>(gamma[0]);
@@ -1249,6 +1284,19 @@ class HipBackend final : public Backend {
     HIPCHK(hipStreamSynchronize(stream_));
   }
 
+  bool correct_inflow(gsk_inflow_state* states, std::size_t n) override {
+    if (n == 0) return true;
+    static_assert(sizeof(gsk_inflow_state) == 6 * sizeof(double), "inflow state is six doubles");
+    begin(kOther);
+    inflow_stage_.upload(reinterpret_cast<const double*>(states), n * 6, stream_);
+    dev::k_inflow_correct<<<static_cast<unsigned>(std::min<std::size_t>(n, 65535)), dev::kBlock, 0, stream_>>>(
+        nv_, xd_.p, yd_.p, zd_.p, weight_, view().deq, inflow_stage_.p, static_cast<int>(n));
+    HIPCHK(hipGetLastError());
+    end();
+    download(reinterpret_cast<double*>(states), inflow_stage_.p, n * 6);
+    return true;
+  }
+
   void boundary_sums(double* out, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nb_) * dev::kBoundarySums, "boundary sums length differs from 20 x faces");
     begin(kBoundary);
@@ -1679,7 +1727,7 @@ class HipBackend final : public Backend {
   DBuf<dev::SweepRec<F>> fw_recs_, bw_recs_;
   DBuf<int> subset_;
   std::vector<int> subset_host_;
-  DBuf<double> bsum_part_;
+  DBuf<double> bsum_part_, inflow_stage_;
   int bsum_chunks_ = 1;
   std::vector<int> vel_list_host_, dir_off_;
   std::vector<int> deq_flags_;
