@@ -123,14 +123,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, case_dir, tube_args, precision, iterations=3):
-    """Oracle (CPU restatement, OpenMP) on the same tube mesh with a 6^3 velocity subset."""
+def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8):
+    """Oracle (CPU restatement, OpenMP) on the same tube mesh with an 8^3 velocity grid (about 10-20 s
+    of CPU work)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import gsis_sim_amd.meshgen as mg
 
     threads = oracle.set_threads(min(16, os.cpu_count() or 1))
-    path = mg.tube_case(case_dir, n_vel=6, vrange=6.0, delta=10.0, kinetic={"precision": precision}, name="cpu_tube",
+    path = mg.tube_case(case_dir, n_vel=n_vel, vrange=6.0, delta=10.0, kinetic={"precision": precision}, name="cpu_tube",
                         **tube_args)
     s = g.Setup(path)
     cb = oracle.CpuBackend(s, "oracle", ops_type=g.BackendOps)
@@ -147,7 +148,7 @@ def cpu_baseline(g, case_dir, tube_args, precision, iterations=3):
     updates = s.n_cells * s.nv * inner * iterations
     sv.close()
     return dict(value=updates / dt / 1e9, unit="Gcell·vel updates/s", cores=threads, kind="port",
-                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a 6^3 velocity grid, "
+                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a {n_vel}^3 velocity grid, "
                        f"{iterations} CIS iterations after 1 untimed ({dt:.1f} s)")
 
 
