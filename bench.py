@@ -247,6 +247,9 @@ def main():
         sv = g.Solver(setup)
         be = sv.backend()
         sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+        free_b, total_b = torch.cuda.mem_get_info()
+        memory = {"backend_bytes": int(be.memory_bytes()), "device_used_bytes": int(total_b - free_b),
+                  "device_total_bytes": int(total_b)}
         passes = setup.view().block_passes
         inner = 4
         for i in range(args.warmup):
@@ -297,6 +300,7 @@ def main():
             "step_roofline": {"bytes_per_update": b_alg, "achieved": round(value * b_alg, 2),
                               "frac": round(value * b_alg / HBM_PEAK_GBS, 4)},
             "phase_seconds_per_step": {k: round(v / args.steps, 5) for k, v in phase.items()},
+            "memory": memory,
         }
         if not args.no_cpu_baseline:
             log("cpu baseline")

@@ -414,9 +414,11 @@ __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int*
 // XCD a few hundred workgroups earlier, with only an eighth of the velocities in between. (Walking
 // all cells of one tile first, with 256- or 64-velocity tiles, reads rows at the stride of a full
 // velocity row and measured slower.) Grid: 8 * ceil(tiles / 8) * n_cells workgroups.
-__device__ __forceinline__ bool xcd_tile(int n_cells, int tiles, int& cell, int& tile) {
-  const int x = static_cast<int>(blockIdx.x % 8);
-  const i64 k = blockIdx.x / 8;
+// L is the work-item index, L0 + blockIdx.x for a launch that starts at L0 (a multiple of 8, so
+// L mod 8 is the launching XCD); a large chunk is covered by several launches.
+__device__ __forceinline__ bool xcd_tile(i64 L, int n_cells, int tiles, int& cell, int& tile) {
+  const int x = static_cast<int>(L % 8);
+  const i64 k = L / 8;
   const int ntx = (tiles - x + 7) / 8;  // tiles owned by this XCD
   if (ntx <= 0) return false;
   const i64 c = k / ntx;
@@ -427,36 +429,73 @@ __device__ __forceinline__ bool xcd_tile(int n_cells, int tiles, int& cell, int&
 }
 
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles) {
+__global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles, i64 L0) {
   int c, tile;
-  if (!xcd_tile(v.n_cells, tiles, c, tile)) return;
+  if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   if (t >= ch.kc) return;
   const int p = ch.k0 + t;
   F gx, gy, gz;
-  limited_gradient(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
+  limited_gradient_ell(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
   const i64 wi = static_cast<i64>(c) * ch.stride + t;
   ch.gx[wi] = gx;
   ch.gy[wi] = gy;
   ch.gz[wi] = gz;
 }
 
+// cells with more than kGather faces, from the CSR (after k_gradient, which wrote them wrongly)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_gradient_csr(View<F> v, Chunk<F> ch, const int* __restrict__ cells, int n,
+                                                         int tiles) {
+  for (i64 L = blockIdx.x; L < static_cast<i64>(n) * tiles; L += gridDim.x) {
+    const int c = cells[L / tiles], tile = static_cast<int>(L % tiles);
+    const int t = tile * kBlock + threadIdx.x;
+    if (t >= ch.kc) continue;
+    const int p = ch.k0 + t;
+    F gx, gy, gz;
+    limited_gradient(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
+    const i64 wi = static_cast<i64>(c) * ch.stride + t;
+    ch.gx[wi] = gx;
+    ch.gy[wi] = gy;
+    ch.gz[wi] = gz;
+  }
+}
+
 // residual R into rres and per (cell, tile) partial sums of R^2 (k_residual)
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int tiles, F* __restrict__ rres,
+__global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int tiles, i64 L0, F* __restrict__ rres,
                                                      double* __restrict__ partial) {
   __shared__ double lds[kWaves];
   int c, tile;
-  if (!xcd_tile(v.n_cells, tiles, c, tile)) return;
+  if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   double r2[1] = {0.0};
   if (t < ch.kc) {
-    const F R = residual_value(v, ch, c, t);
+    const F R = residual_value_ell(v, ch, c, t);
     rres[static_cast<i64>(c) * ch.stride + t] = R;
     r2[0] = static_cast<double>(R) * static_cast<double>(R);
   }
   block_sum<1>(r2, lds);
   if (threadIdx.x == 0) partial[static_cast<i64>(c) * tiles + tile] = r2[0];
+}
+
+// cells with more than kGather faces, from the CSR (after k_residual, which wrote them wrongly)
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_residual_csr(View<F> v, Chunk<F> ch, const int* __restrict__ cells, int n,
+                                                         int tiles, F* __restrict__ rres, double* __restrict__ partial) {
+  __shared__ double lds[kWaves];
+  for (i64 L = blockIdx.x; L < static_cast<i64>(n) * tiles; L += gridDim.x) {
+    const int c = cells[L / tiles], tile = static_cast<int>(L % tiles);
+    const int t = tile * kBlock + threadIdx.x;
+    double r2[1] = {0.0};
+    if (t < ch.kc) {
+      const F R = residual_value(v, ch, c, t);
+      rres[static_cast<i64>(c) * ch.stride + t] = R;
+      r2[0] = static_cast<double>(R) * static_cast<double>(R);
+    }
+    block_sum<1>(r2, lds);
+    if (threadIdx.x == 0) partial[static_cast<i64>(c) * tiles + tile] = r2[0];
+  }
 }
 
 __global__ void k_residual_reduce(const double* __restrict__ partial, int tiles, int n_cells, double* __restrict__ out) {
@@ -1065,12 +1104,37 @@ class HipBackend final : public Backend {
                 "CSR entry refers to a cell or face out of range");
     }
     csr_.upload(csr.data(), csr.size(), stream_);
+    // ELL copy: kGather entries per cell, padded with entries on the no-op periodic face nb_ (appended
+    // to the boundary-face table below); cells with more faces are listed for the CSR path
+    {
+      dev::CsrE<F> pad{};
+      pad.other = -1;
+      pad.bface = nb_;
+      std::vector<dev::CsrE<F>> ell(static_cast<std::size_t>(nc_) * dev::kGather, pad);
+      std::vector<int> irregular;
+      // test hook: GSK_ELL_MAX_FACES lowers the face count above which a cell takes the CSR path
+      int ell_max = dev::kGather;
+      if (const char* env = std::getenv("GSK_ELL_MAX_FACES")) ell_max = std::max(0, std::min(dev::kGather, std::atoi(env)));
+      if (const char* env = std::getenv("GSK_GRID_CAP")) grid_cap_ = std::max(8LL, std::min(grid_cap_, std::atoll(env)) / 8 * 8);
+      for (int c = 0; c < nc_; ++c) {
+        const int e0 = s.csr_offsets[c], e1 = s.csr_offsets[c + 1];
+        if (e1 - e0 > ell_max) {
+          irregular.push_back(c);
+          continue;
+        }
+        for (int k = e0; k < e1 && k - e0 < dev::kGather; ++k)
+          ell[static_cast<std::size_t>(c) * dev::kGather + static_cast<std::size_t>(k - e0)] = csr[static_cast<std::size_t>(k)];
+      }
+      ell_.upload(ell.data(), ell.size(), stream_);
+      n_irregular_ = static_cast<int>(irregular.size());
+      irregular_.upload(irregular.data(), irregular.size(), stream_);
+    }
     std::vector<F> eps2(static_cast<std::size_t>(nc_));
     for (int c = 0; c < nc_; ++c) eps2[static_cast<std::size_t>(c)] = static_cast<F>(s.eps2[c]);
     eps2_.upload(eps2.data(), eps2.size(), stream_);
     volume_.upload(s.volume, static_cast<std::size_t>(nc_), stream_);
     vdt_.upload(s.vdt, static_cast<std::size_t>(nc_), stream_);
-    std::vector<dev::BFace<F>> bf(static_cast<std::size_t>(nb_));
+    std::vector<dev::BFace<F>> bf(static_cast<std::size_t>(nb_) + 1);  // + the padding face
     for (int b = 0; b < nb_; ++b) {
       const gsk_bface& i = s.bfaces[b];
       dev::BFace<F>& o = bf[static_cast<std::size_t>(b)];
@@ -1088,6 +1152,7 @@ class HipBackend final : public Backend {
       GSK_CHECK(i.owner >= 0 && i.owner < nc_, "boundary face owner out of range");
       GSK_CHECK(i.mirror_xor >= 0 && i.mirror_xor < nv_, "boundary face mirror out of range");
     }
+    bf[static_cast<std::size_t>(nb_)].kind = dev::kBcPeriodic;
     bfaces_.upload(bf.data(), bf.size(), stream_);
     host_bface_kind_.resize(static_cast<std::size_t>(nb_));
     for (int b = 0; b < nb_; ++b) host_bface_kind_[static_cast<std::size_t>(b)] = s.bfaces[b].kind;
@@ -1103,6 +1168,7 @@ class HipBackend final : public Backend {
     build_levels(s);
     // Maxwell walls
     n_mw_ = s.n_maxwell_faces;
+    GSK_CHECK(n_mw_ == 0 || nb_ <= 65535, "Maxwell-wall reflection launches take at most 65535 boundary faces");
     if (n_mw_ > 0) {
       mw_off_.upload(reinterpret_cast<const long long*>(s.maxwell_offset), static_cast<std::size_t>(nb_), stream_);
       std::vector<F> frac(static_cast<std::size_t>(nb_));
@@ -1339,21 +1405,35 @@ class HipBackend final : public Backend {
     dev::k_boundary_reduce<<<(nj + 255) / 256, 256, 0, stream_>>>(faces, n_faces, bsum_chunks_, bsum_part_.p, bsums_.p);
   }
 
+  // Workgroups of one launch over `work` items: at most grid_cap_ (a multiple of 8, so a kernel's
+  // L mod 8 stays its XCD), keeping blocks x threads far below the 2^32 a launch allows; the
+  // gradient / residual cover larger work with several launches, the CSR-path kernels loop.
+  unsigned capped_grid(long long work) const {
+    const long long g = std::min<long long>(std::max<long long>(work, 8), grid_cap_);
+    return static_cast<unsigned>((g + 7) / 8 * 8);
+  }
+
   void sweep(double* residual_cell, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
     HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
     const dev::View<F> v = view();
     for (int k0 = 0; k0 < nv_; k0 += chunk_) {
       dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
-      const long long grid_ll = 8LL * ((tiles_ + 7) / 8) * nc_;
-      GSK_CHECK(grid_ll < (1LL << 32), "grid too large");
-      const unsigned grid = static_cast<unsigned>(grid_ll);
+      const long long work = 8LL * ((tiles_ + 7) / 8) * nc_;
+      const unsigned grid_irr = capped_grid(static_cast<long long>(n_irregular_) * tiles_);
       begin(kGradient);
-      dev::k_gradient<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_);
+      for (long long L0 = 0; L0 < work; L0 += grid_cap_)
+        dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0);
+      if (n_irregular_ > 0)
+        dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
       HIPCHK(hipGetLastError());
       end();
       begin(kResidual);
-      dev::k_residual<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, rres_.p, partial_.p);
+      for (long long L0 = 0; L0 < work; L0 += grid_cap_)
+        dev::k_residual<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0, rres_.p, partial_.p);
+      if (n_irregular_ > 0)
+        dev::k_residual_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_,
+                                                                     rres_.p, partial_.p);
       dev::k_residual_reduce<<<(nc_ + 255) / 256, 256, 0, stream_>>>(partial_.p, tiles_, nc_, residual_.p);
       HIPCHK(hipGetLastError());
       end();
@@ -1370,7 +1450,7 @@ class HipBackend final : public Backend {
       const int tiles = std::max(max_tiles, 1);
       const int n_pairs = n_dirs_eff_ * n_blocks_;
       const long long sgrid_ll = 8LL * ((n_pairs + 7) / 8) * tiles;
-      GSK_CHECK(sgrid_ll < (1LL << 31), "sweep grid too large");
+      GSK_CHECK(sgrid_ll * 64 * waves_ < (1LL << 32), "sweep grid too large");
       const unsigned sgrid = static_cast<unsigned>(sgrid_ll);
       const dev::SweepArgs fw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, fw_stage_.p,
                               fw_group_.p, vr};
@@ -1670,6 +1750,7 @@ class HipBackend final : public Backend {
     v.weight_d = weight_;
     v.csr_off = csr_off_.p;
     v.csr = csr_.p;
+    v.ell = ell_.p;
     v.eps2 = eps2_.p;
     v.limiter_mode = limiter_mode_;
     v.positivity = positivity_;
@@ -1706,7 +1787,10 @@ class HipBackend final : public Backend {
   DBuf<F> xi_x_, xi_y_, xi_z_;
   DBuf<double> xd_, yd_, zd_;
   DBuf<int> csr_off_;
-  DBuf<dev::CsrE<F>> csr_;
+  DBuf<dev::CsrE<F>> csr_, ell_;
+  DBuf<int> irregular_;
+  int n_irregular_ = 0;
+  long long grid_cap_ = 8LL << 20;  // workgroups per launch; GSK_GRID_CAP lowers it (test hook)
   DBuf<F> eps2_;
   DBuf<double> volume_, vdt_;
   DBuf<dev::BFace<F>> bfaces_;

@@ -42,6 +42,7 @@ struct View {
   double weight_d;
   const int* csr_off;
   const CsrE<F>* csr;
+  const CsrE<F>* ell;  // [cell][kGather] copy of the CSR entries, padded with a no-op periodic face
   const F* eps2;
   int limiter_mode, positivity;
   F diag_scale, interface_damping;
@@ -102,8 +103,8 @@ __device__ F boundary_value_first_order(const View<F>& v, const CsrE<F>& e, int 
 constexpr int kGather = 6;  // faces held in registers (hex 6, prism/pyramid 5, tet 4); more: CSR path
 
 template <class F>
-__device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, int e0, int e1, F fi, F fmx, F fmn, F gx,
-                                                F gy, F gz, F& ogx, F& ogy, F& ogz) {
+__device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, const CsrE<F>* ents, int n, F fi, F fmx,
+                                                F fmn, F gx, F gy, F gz, F& ogx, F& ogy, F& ogz) {
   if (v.limiter_mode == kLimNone && v.positivity == 0) {
     ogx = gx;
     ogy = gy;
@@ -122,8 +123,8 @@ __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, int 
   const F dmx = fmx - fi, dmn = fmn - fi;
   F phi = F(1);
   F proj_min = F(0);
-  for (int k = e0; k < e1; ++k) {
-    const CsrE<F>& e = v.csr[k];
+  for (int k = 0; k < n; ++k) {
+    const CsrE<F>& e = ents[k];
     if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
     const F proj = gx * e.dsx + gy * e.dsy + gz * e.dsz;
     if (v.limiter_mode != kLimNone) phi = dmin(phi, venkat_face(proj, dmx, dmn, eps2));
@@ -135,44 +136,66 @@ __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, int 
   ogz = phi * gz;
 }
 
+// gradient from at most kGather entries ents[0..n): neighbour values gathered first, then combined
+// in entry order
 template <class F>
-__device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
-  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+__device__ __forceinline__ void gradient_gathered(const View<F>& v, int cell, int p, F x, F y, F z, F fi,
+                                                  const CsrE<F>* ents, int n, F& ogx, F& ogy, F& ogz) {
   F gx = F(0), gy = F(0), gz = F(0);
   F fmx = fi, fmn = fi;
-  if (e1 - e0 <= kGather) {
-    F fk[kGather];
+  F fk[kGather];
 #pragma unroll
-    for (int k = 0; k < kGather; ++k)
-      if (k < e1 - e0) {
-        const int o = v.csr[e0 + k].other;
-        fk[k] = o >= 0 ? f_at(v, o, p) : F(0);
-      }
+  for (int k = 0; k < kGather; ++k)
+    if (k < n) {
+      const int o = ents[k].other;
+      fk[k] = o >= 0 ? f_at(v, o, p) : F(0);
+    }
 #pragma unroll
-    for (int k = 0; k < kGather; ++k)
-      if (k < e1 - e0) {
-        const CsrE<F>& e = v.csr[e0 + k];
-        if (e.other < 0) fk[k] = boundary_value_first_order(v, e, cell, p, x, y, z, fi);
-        const F d = fk[k] - fi;
-        gx += e.cx * d;
-        gy += e.cy * d;
-        gz += e.cz * d;
-        fmx = dmax(fmx, fk[k]);
-        fmn = dmin(fmn, fk[k]);
-      }
-  } else {
-    for (int k = e0; k < e1; ++k) {
-      const CsrE<F>& e = v.csr[k];
-      const F fk = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
-      const F d = fk - fi;
+  for (int k = 0; k < kGather; ++k)
+    if (k < n) {
+      const CsrE<F>& e = ents[k];
+      if (e.other < 0) fk[k] = boundary_value_first_order(v, e, cell, p, x, y, z, fi);
+      const F d = fk[k] - fi;
       gx += e.cx * d;
       gy += e.cy * d;
       gz += e.cz * d;
-      fmx = dmax(fmx, fk);
-      fmn = dmin(fmn, fk);
+      fmx = dmax(fmx, fk[k]);
+      fmn = dmin(fmn, fk[k]);
     }
+  limit_and_scale(v, cell, ents, n, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
+}
+
+template <class F>
+__device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
+  const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
+  if (e1 - e0 <= kGather) {
+    gradient_gathered(v, cell, p, x, y, z, fi, v.csr + e0, e1 - e0, ogx, ogy, ogz);
+    return;
   }
-  limit_and_scale(v, cell, e0, e1, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
+  F gx = F(0), gy = F(0), gz = F(0);
+  F fmx = fi, fmn = fi;
+  for (int k = e0; k < e1; ++k) {
+    const CsrE<F>& e = v.csr[k];
+    const F fk = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
+    const F d = fk - fi;
+    gx += e.cx * d;
+    gy += e.cy * d;
+    gz += e.cz * d;
+    fmx = dmax(fmx, fk);
+    fmn = dmin(fmn, fk);
+  }
+  limit_and_scale(v, cell, v.csr + e0, e1 - e0, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
+}
+
+// The same gradient from the cell's padded row of the ELL table: the entry count is the constant
+// kGather, so the entry loads issue together with no branch on the face count. The padding entries
+// name a periodic face: first-order value fi (d = 0, adds +0 to sums that start at +0 and so never
+// hold -0; fmx / fmn unchanged), skipped by the limiter. Cells with more faces get a wrong value
+// here and are recomputed by the CSR path afterwards.
+template <class F>
+__device__ __forceinline__ void limited_gradient_ell(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx,
+                                                     F& ogy, F& ogz) {
+  gradient_gathered(v, cell, p, x, y, z, fi, v.ell + static_cast<i64>(cell) * kGather, kGather, ogx, ogy, ogz);
 }
 
 // outgoing reconstructed value at a boundary face (kinetic_kernels.hpp:204-211)
@@ -254,8 +277,64 @@ __device__ F internal_face_flux_term(const View<F>& v, int cell, const CsrE<F>& 
   return a * ff;
 }
 
-// steady residual R = sum_e a f_e - coll (f_S - f_i) (kinetic_kernels.hpp:238-278); upwind neighbour
-// values (f and its limited gradient) are gathered before the flux sum, which runs in CSR order.
+// Flux sum over at most kGather entries ents[0..n) (kinetic_kernels.hpp:238-272): upwind neighbour
+// values (f and its limited gradient) are gathered first, then the fluxes are added in entry order.
+template <class F>
+__device__ __forceinline__ F flux_gathered(const View<F>& v, const Chunk<F>& ch, int cell, int t, int p, F x, F y, F z,
+                                           F fi, F gix, F giy, F giz, const CsrE<F>* ents, int n) {
+  F R = F(0);
+  F ak[kGather], fu[kGather], gux[kGather], guy[kGather], guz[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < n) {
+      const CsrE<F>& e = ents[k];
+      ak[k] = x * e.sx + y * e.sy + z * e.sz;
+      fu[k] = gux[k] = guy[k] = guz[k] = F(0);
+      if (ak[k] < F(0)) {
+        int src_cell = -1, src_t = t;
+        if (e.other >= 0) {
+          src_cell = e.other;
+        } else if (v.bf[e.bface].kind == kBcSymmetry) {
+          src_cell = cell;
+          src_t = t ^ v.bf[e.bface].mirror_xor;
+        }
+        if (src_cell >= 0) {
+          const i64 wj = static_cast<i64>(src_cell) * ch.stride + src_t;
+          fu[k] = f_at(v, src_cell, ch.k0 + src_t);
+          gux[k] = ch.gx[wj];
+          guy[k] = ch.gy[wj];
+          guz[k] = ch.gz[wj];
+        }
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < kGather; ++k)
+    if (k < n) {
+      const CsrE<F>& e = ents[k];
+      int kind = -1;
+      if (e.other < 0) {
+        kind = v.bf[e.bface].kind;
+        if (kind == kBcPeriodic) continue;
+      }
+      const F a = ak[k];
+      F ff;
+      if (a >= F(0)) {
+        ff = fi + gix * e.dsx + giy * e.dsy + giz * e.dsz;
+      } else if (e.other >= 0) {
+        ff = fu[k] + gux[k] * e.dox + guy[k] * e.doy + guz[k] * e.doz;
+      } else if (kind == kBcSymmetry) {
+        ff = fu[k] + gux[k] * e.dsx + guy[k] * e.dsy + guz[k] * e.dsz;
+      } else if (kind == kBcMaxwell) {
+        ff = maxwell_inflow(v, e.bface, p, x, y, z);
+      } else {
+        ff = inflow_value(v.inflow[e.bface], x, y, z);
+      }
+      R += a * ff;
+    }
+  return R;
+}
+
+// steady residual R = sum_e a f_e - coll (f_S - f_i) (kinetic_kernels.hpp:238-278)
 template <class F>
 __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int t) {
   const int p = ch.k0 + t;
@@ -266,54 +345,7 @@ __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int 
   F R = F(0);
   const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
   if (e1 - e0 <= kGather) {
-    F ak[kGather], fu[kGather], gux[kGather], guy[kGather], guz[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; ++k)
-      if (k < e1 - e0) {
-        const CsrE<F>& e = v.csr[e0 + k];
-        ak[k] = x * e.sx + y * e.sy + z * e.sz;
-        fu[k] = gux[k] = guy[k] = guz[k] = F(0);
-        if (ak[k] < F(0)) {
-          int src_cell = -1, src_t = t;
-          if (e.other >= 0) {
-            src_cell = e.other;
-          } else if (v.bf[e.bface].kind == kBcSymmetry) {
-            src_cell = cell;
-            src_t = t ^ v.bf[e.bface].mirror_xor;
-          }
-          if (src_cell >= 0) {
-            const i64 wj = static_cast<i64>(src_cell) * ch.stride + src_t;
-            fu[k] = f_at(v, src_cell, ch.k0 + src_t);
-            gux[k] = ch.gx[wj];
-            guy[k] = ch.gy[wj];
-            guz[k] = ch.gz[wj];
-          }
-        }
-      }
-#pragma unroll
-    for (int k = 0; k < kGather; ++k)
-      if (k < e1 - e0) {
-        const CsrE<F>& e = v.csr[e0 + k];
-        int kind = -1;
-        if (e.other < 0) {
-          kind = v.bf[e.bface].kind;
-          if (kind == kBcPeriodic) continue;
-        }
-        const F a = ak[k];
-        F ff;
-        if (a >= F(0)) {
-          ff = fi + gix * e.dsx + giy * e.dsy + giz * e.dsz;
-        } else if (e.other >= 0) {
-          ff = fu[k] + gux[k] * e.dox + guy[k] * e.doy + guz[k] * e.doz;
-        } else if (kind == kBcSymmetry) {
-          ff = fu[k] + gux[k] * e.dsx + guy[k] * e.dsy + guz[k] * e.dsz;
-        } else if (kind == kBcMaxwell) {
-          ff = maxwell_inflow(v, e.bface, p, x, y, z);
-        } else {
-          ff = inflow_value(v.inflow[e.bface], x, y, z);
-        }
-        R += a * ff;
-      }
+    R = flux_gathered(v, ch, cell, t, p, x, y, z, fi, gix, giy, giz, v.csr + e0, e1 - e0);
   } else {
     for (int k = e0; k < e1; ++k) {
       const CsrE<F>& e = v.csr[k];
@@ -341,6 +373,21 @@ __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int 
       R += a * ff;
     }
   }
+  const CellP<F>& cp = v.cells[cell];
+  R -= cp.coll * (target_eval(cp.s, x, y, z) - fi);
+  return R;
+}
+
+// The same residual from the cell's padded ELL row (see limited_gradient_ell); padding entries are
+// periodic faces, which the flux sum skips.
+template <class F>
+__device__ F residual_value_ell(const View<F>& v, const Chunk<F>& ch, int cell, int t) {
+  const int p = ch.k0 + t;
+  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
+  const F fi = f_at(v, cell, p);
+  const i64 wi = static_cast<i64>(cell) * ch.stride + t;
+  const F gix = ch.gx[wi], giy = ch.gy[wi], giz = ch.gz[wi];
+  F R = flux_gathered(v, ch, cell, t, p, x, y, z, fi, gix, giy, giz, v.ell + static_cast<i64>(cell) * kGather, kGather);
   const CellP<F>& cp = v.cells[cell];
   R -= cp.coll * (target_eval(cp.s, x, y, z) - fi);
   return R;

@@ -126,6 +126,35 @@ def test_solver_iterations_match_oracle(gsk, meshgen, oracle_mod, tmp_path, name
     assert np.allclose(gpu["flows"], cpu["flows"], rtol=1e-10, atol=1e-12 * max(1.0, np.max(np.abs(cpu["flows"]))))
 
 
+@pytest.mark.parametrize("name", ["box_mixed", "tube_curved_maxwell"])
+def test_csr_path_cells_match_oracle(gsk, meshgen, oracle_mod, tmp_path, monkeypatch, name):
+    """Cells with more faces than the gathered (ELL) kernels hold are recomputed from the CSR after
+    those kernels; GSK_ELL_MAX_FACES=0 sends every cell down that path (the box mixes hex cells
+    with boundary faces of every kind)."""
+    print(f"[gpu] CSR-path cells {name}", flush=True)
+    monkeypatch.setenv("GSK_ELL_MAX_FACES", "0")
+    setup = gsk.Setup(SOLVER_CASES[name](meshgen, str(tmp_path)))
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+    cpu = _run_solver(gsk, setup, cb.ops, 4)
+    gpu = _run_solver(gsk, setup, None, 4)
+    assert rel_err(gpu["f"], cpu["f"]) <= 1e-10
+    assert rel_err(gpu["hist"][:, 0], cpu["hist"][:, 0]) <= 1e-10
+
+
+def test_split_launches_match_oracle(gsk, meshgen, oracle_mod, tmp_path, monkeypatch):
+    """Large meshes exceed what one launch may hold: the gradient / residual then take several
+    launches and the CSR-path kernels loop; GSK_GRID_CAP=16 forces both on a small mesh."""
+    print("[gpu] split launches box_mixed", flush=True)
+    monkeypatch.setenv("GSK_GRID_CAP", "16")
+    monkeypatch.setenv("GSK_ELL_MAX_FACES", "5")
+    setup = gsk.Setup(SOLVER_CASES["box_mixed"](meshgen, str(tmp_path)))
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+    cpu = _run_solver(gsk, setup, cb.ops, 3)
+    gpu = _run_solver(gsk, setup, None, 3)
+    assert rel_err(gpu["f"], cpu["f"]) <= 1e-10
+    assert rel_err(gpu["hist"][:, 0], cpu["hist"][:, 0]) <= 1e-10
+
+
 def test_slab_benchmark_300_iterations_match_golden(gsk, meshgen, tmp_path):
     """The 1D slab benchmark (200 cells, 24^3 on [-6, 6], delta = 1, 300 CIS iterations, FP64) against
     the committed oracle vectors (tests/golden/slab_c1.npz): moments within 1e-10 relative."""
