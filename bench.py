@@ -39,10 +39,13 @@ def _package():
 
 
 WORKLOADS = {
-    # name: (tube args, velocity points per axis, precision)
-    "tube40k_24cubed_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, "fp64"),
-    "tube40k_24cubed_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, "fp32"),
-    "tube135k_32cubed_fp32": (dict(n_core=24, n_ring=14, n_axial=70), 32, "fp32"),
+    # name: (tube args, velocity points per axis, [kinetic] settings); "mixed" = FP32 f with the
+    # matrix-core moment contraction
+    "tube40k_24cubed_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp64"}),
+    "tube40k_24cubed_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32"}),
+    "tube40k_24cubed_mixed": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32", "moment_sums": "mfma"}),
+    "tube135k_32cubed_fp32": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32"}),
+    "tube135k_32cubed_mixed": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32", "moment_sums": "mfma"}),
 }
 
 # algorithmic bytes per (cell, velocity) pair and launch, in units of sizeof(f)
@@ -160,8 +163,9 @@ def time_to_mdot(g, mg, args):
     tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
     runs = {}
     with tempfile.TemporaryDirectory() as d:
-        for prec in ("fp64", "fp32"):
-            path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=10.0, kinetic={"precision": prec}, name=f"tube_{prec}",
+        for prec, kin in (("fp64", {"precision": "fp64"}), ("fp32", {"precision": "fp32"}),
+                          ("mixed", {"precision": "fp32", "moment_sums": "mfma"})):
+            path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=10.0, kinetic=kin, name=f"tube_{prec}",
                                 **tube_args)
             setup = g.Setup(path)
             sv = g.Solver(setup)
@@ -182,6 +186,8 @@ def time_to_mdot(g, mg, args):
                      "time_to_mdot_s": None if t is None else t[0], "iterations_to_mdot": None if t is None else t[1]}
     if out["fp32"]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
         out["speedup_fp32_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out["fp32"]["time_to_mdot_s"]
+    if out["mixed"]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
+        out["speedup_mixed_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out["mixed"]["time_to_mdot_s"]
     print(json.dumps(out), flush=True)
 
 
@@ -237,11 +243,12 @@ def main():
     if args.time_to_mdot:
         time_to_mdot(g, mg, args)
         return
-    tube_args, nvel, precision = WORKLOADS[args.workload]
+    tube_args, nvel, kinetic = WORKLOADS[args.workload]
+    precision = kinetic["precision"]
     s_bytes = 8 if precision == "fp64" else 4
     with tempfile.TemporaryDirectory() as d:
         log(f"generating {args.workload}")
-        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=10.0, kinetic={"precision": precision}, **tube_args)
+        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=10.0, kinetic=dict(kinetic), **tube_args)
         setup = g.Setup(path)
         log(f"setup: {setup.n_cells} cells, {setup.nv} velocities, {setup.n_blocks} blocks")
         sv = g.Solver(setup)

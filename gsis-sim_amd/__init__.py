@@ -72,6 +72,7 @@ class SetupView(C.Structure):
         ("positivity", C.c_int32), ("diag_scale", C.c_double), ("interface_damping", C.c_double),
         ("cold_bgk_temperature", C.c_double), ("mu0", C.c_double), ("omega", C.c_double), ("Pr", C.c_double),
         ("block_passes", C.c_int32), ("negative_check", C.c_int32), ("precision_fp64", C.c_int32),
+        ("moments_mfma", C.c_int32),
     ]
 
 

@@ -489,6 +489,7 @@ KineticParams parse_kinetic_params(const TomlTable& t) {
   integer("sweep_blocks", p.sweep_blocks);
   str("cell_ordering", p.cell_ordering);
   str("precision", p.precision);
+  str("moment_sums", p.moment_sums);
   str("backend", p.backend);
   integer("max_iterations", p.max_iterations);
   num("initial_pressure", p.initial_pressure);
@@ -529,6 +530,9 @@ void validate_kinetic_params(const KineticParams& p) {
   GSK_CHECK(p.sweep_blocks >= 0, "kinetic.sweep_blocks must be >= 0");
   GSK_CHECK(one_of(p.cell_ordering, {"rcb", "rcm", "none"}), "kinetic.cell_ordering must be rcb, rcm or none");
   GSK_CHECK(one_of(p.precision, {"fp32", "fp64"}), "kinetic.precision must be \"fp32\" or \"fp64\"");
+  GSK_CHECK(one_of(p.moment_sums, {"fp64", "mfma"}), "kinetic.moment_sums must be \"fp64\" or \"mfma\"");
+  GSK_CHECK(p.moment_sums == "fp64" || p.precision == "fp32",
+            "kinetic.moment_sums = \"mfma\" needs precision = \"fp32\" (the FP64 form sums in FP64)");
   GSK_CHECK(one_of(p.backend, {"cuda", "hip", "cpu"}), "kinetic.backend must be \"cuda\", \"hip\" or \"cpu\"");
   GSK_CHECK(p.max_iterations >= 1, "kinetic.max_iterations must be >= 1");
   GSK_CHECK(p.initial_pressure > 0.0, "kinetic.initial_pressure must be positive");

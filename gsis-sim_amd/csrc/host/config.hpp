@@ -99,6 +99,7 @@ struct KineticParams {
   long long sweep_blocks = 0;
   std::string cell_ordering = "rcb";
   std::string precision = "fp32";
+  std::string moment_sums = "fp64";  // extension: "mfma" = matrix-core moment contraction (FP32 form)
   std::string backend = "cuda";
   long long max_iterations = 100000;
   double initial_pressure = 1.0;

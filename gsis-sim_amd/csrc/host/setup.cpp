@@ -471,6 +471,7 @@ gsk_setup_view KineticSetup::view() const {
   const std::string& rule = params.gsis_negative_check;
   v.negative_check = rule == "created" ? 1 : rule == "limit" ? 2 : rule == "blend" ? 3 : 0;
   v.precision_fp64 = params.precision == "fp64" ? 1 : 0;
+  v.moments_mfma = params.moment_sums == "mfma" ? 1 : 0;
   return v;
 }
 

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../host/backend.hpp"
@@ -257,6 +258,107 @@ __global__ __launch_bounds__(kBlock) void k_moments(View<F> v, const double* __r
         shift[c * 3 + 1] = m[2];
         shift[c * 3 + 2] = m[3];
       }
+    }
+  }
+}
+
+// Moment sums of the FP32 form as a dense contraction on the matrix cores (an extension; the FP64
+// form and the reference sum in FP64, k_moments). For a tile of 16 cells (one workgroup),
+// D[m][n] = sum_k psi_m(xi_k - s) f[n][k], psi = the 13 monomials of moment_terms (1, c, cc, |c|^2 c;
+// rows 13..15 zero) about one shift s per tile: the mean of the tile's cell shifts, rounded to FP32.
+// FP32 products and accumulation on v_mfma_f32_16x16x4f32 (A: lane l holds A[l%16][l/16], B: lane l
+// holds B[l/16][l%16], D: lane l register i holds D[4(l/16)+i][l%16]). The 4 waves take a quarter of
+// the velocities each; FP32 accumulators are added into FP64 every kMfmaBlock velocities, the waves'
+// FP64 partials are added in wave order, and finalize_moments forms the moments about s in FP64.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaBlock = 512;
+__global__ __launch_bounds__(256) void k_moments_mfma(View<float> v, double* __restrict__ shift,
+                                                      double* __restrict__ moments) {
+  __shared__ double part[4][16][16];  // [wave][cell][moment row]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 16;
+  const int n = lane & 15, g = lane >> 4;
+  const int nvalid = min(16, v.n_cells - c0);
+  const i64 c = c0 + min(n, nvalid - 1);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int j = 0; j < nvalid; ++j) {
+    sx += shift[(c0 + j) * 3 + 0];
+    sy += shift[(c0 + j) * 3 + 1];
+    sz += shift[(c0 + j) * 3 + 2];
+  }
+  const float fsx = static_cast<float>(sx / nvalid), fsy = static_cast<float>(sy / nvalid),
+              fsz = static_cast<float>(sz / nvalid);
+  // this lane's monomial (row m = n): e1 * e2 * e3 with factors chosen by axis (0 = none)
+  const int m = n;
+  const int ax1 = (m == 1 || m == 4 || m == 7 || m == 9 || m == 10) ? 1
+                  : (m == 2 || m == 5 || m == 8 || m == 11)         ? 2
+                  : (m == 3 || m == 6 || m == 12)                   ? 3
+                                                                    : 0;
+  const int ax2 = (m == 4) ? 1 : (m == 5 || m == 7) ? 2 : (m == 6 || m == 8 || m == 9) ? 3 : 0;
+  const bool cube = m >= 10 && m <= 12;
+  const float base = m < 13 ? 1.0f : 0.0f;
+  // velocity segment of this wave (multiple of 16 long)
+  const int seg = ((v.nv + 63) / 64) * 16;
+  const int k_begin = wave * seg, k_end = min(v.nv, k_begin + seg);
+  const float* frow = v.f + c * v.nv;
+  const bool vec4 = (v.nv % 4) == 0;  // rows 16-byte aligned: float4 loads
+  double acc64[4] = {0.0, 0.0, 0.0, 0.0};
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = k_begin; k0 < k_end; k0 += 16) {
+    const int pb = k0 + 4 * g;  // this lane's 4 velocities pb .. pb + 3 (one per MFMA)
+    float fv[4], xv[4], yv[4], zv[4];
+    if (vec4 && pb + 3 < k_end) {
+      const float4 f4 = *reinterpret_cast<const float4*>(frow + pb);
+      const float4 x4 = *reinterpret_cast<const float4*>(v.xi_x + pb);
+      const float4 y4 = *reinterpret_cast<const float4*>(v.xi_y + pb);
+      const float4 z4 = *reinterpret_cast<const float4*>(v.xi_z + pb);
+      fv[0] = f4.x, fv[1] = f4.y, fv[2] = f4.z, fv[3] = f4.w;
+      xv[0] = x4.x, xv[1] = x4.y, xv[2] = x4.z, xv[3] = x4.w;
+      yv[0] = y4.x, yv[1] = y4.y, yv[2] = y4.z, yv[3] = y4.w;
+      zv[0] = z4.x, zv[1] = z4.y, zv[2] = z4.z, zv[3] = z4.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = pb + j;
+        const bool ok = p < k_end;
+        const int q = ok ? p : k_begin;
+        fv[j] = ok ? frow[q] : 0.0f;
+        xv[j] = v.xi_x[q];
+        yv[j] = v.xi_y[q];
+        zv[j] = v.xi_z[q];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float cx = xv[j] - fsx, cy = yv[j] - fsy, cz = zv[j] - fsz;
+      const float e1 = ax1 == 1 ? cx : ax1 == 2 ? cy : ax1 == 3 ? cz : base;
+      const float e2 = ax2 == 1 ? cx : ax2 == 2 ? cy : ax2 == 3 ? cz : 1.0f;
+      const float e3 = cube ? cx * cx + cy * cy + cz * cz : 1.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(e1 * e2 * e3, fv[j], acc, 0, 0, 0);
+    }
+    if ((k0 + 16 - k_begin) % kMfmaBlock == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        acc64[i] += static_cast<double>(acc[i]);
+        acc[i] = 0.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) part[wave][n][4 * g + i] = acc64[i] + static_cast<double>(acc[i]);
+  __syncthreads();
+  if (threadIdx.x < nvalid) {
+    double s[kMomentSums];
+#pragma unroll
+    for (int r = 0; r < kMomentSums; ++r)
+      s[r] = ((part[0][threadIdx.x][r] + part[1][threadIdx.x][r]) + part[2][threadIdx.x][r]) + part[3][threadIdx.x][r];
+    const i64 cc = c0 + threadIdx.x;
+    double* mo = moments + cc * kMoments;
+    finalize_moments(s, v.weight_d, static_cast<double>(fsx), static_cast<double>(fsy), static_cast<double>(fsz), mo);
+    if (mo[0] > 0.0) {
+      shift[cc * 3 + 0] = mo[1];
+      shift[cc * 3 + 1] = mo[2];
+      shift[cc * 3 + 2] = mo[3];
     }
   }
 }
@@ -1168,6 +1270,8 @@ class HipBackend final : public Backend {
     build_levels(s);
     // Maxwell walls
     n_mw_ = s.n_maxwell_faces;
+    moments_mfma_ = s.moments_mfma != 0;
+    GSK_CHECK(!moments_mfma_ || (std::is_same_v<F, float>), "matrix-core moment sums are for the FP32 form");
     GSK_CHECK(n_mw_ == 0 || nb_ <= 65535, "Maxwell-wall reflection launches take at most 65535 boundary faces");
     if (n_mw_ > 0) {
       mw_off_.upload(reinterpret_cast<const long long*>(s.maxwell_offset), static_cast<std::size_t>(nb_), stream_);
@@ -1300,7 +1404,14 @@ class HipBackend final : public Backend {
   void compute_moments(double* out, std::size_t n) override {
     GSK_CHECK(n == moments_.n, "moments output length differs from 15 x cells");
     begin(kMoments);
-    dev::k_moments<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), xd_.p, yd_.p, zd_.p, shift_.p, moments_.p);
+    if constexpr (std::is_same_v<F, float>) {
+      if (moments_mfma_)
+        dev::k_moments_mfma<<<static_cast<unsigned>((nc_ + 15) / 16), 256, 0, stream_>>>(view(), shift_.p, moments_.p);
+      else
+        dev::k_moments<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), xd_.p, yd_.p, zd_.p, shift_.p, moments_.p);
+    } else {
+      dev::k_moments<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), xd_.p, yd_.p, zd_.p, shift_.p, moments_.p);
+    }
     HIPCHK(hipGetLastError());
     end();
     download(out, moments_.p, n);
@@ -1790,6 +1901,7 @@ class HipBackend final : public Backend {
   DBuf<dev::CsrE<F>> csr_, ell_;
   DBuf<int> irregular_;
   int n_irregular_ = 0;
+  bool moments_mfma_ = false;
   long long grid_cap_ = 8LL << 20;  // workgroups per launch; GSK_GRID_CAP lowers it (test hook)
   DBuf<F> eps2_;
   DBuf<double> volume_, vdt_;

@@ -121,6 +121,8 @@ typedef struct gsk_setup_view {
   int32_t block_passes;
   int32_t negative_check;       /* 0 all, 1 created, 2 limit, 3 blend */
   int32_t precision_fp64;       /* 1: f stored and swept in FP64; 0: FP32 */
+  int32_t moments_mfma;         /* [kinetic].moment_sums == "mfma" (FP32 form only): moment sums as an
+                                   FP32 matrix-core contraction, added in FP64 (no reference counterpart) */
 } gsk_setup_view;
 
 const char* gsk_last_error(void);

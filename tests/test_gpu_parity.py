@@ -202,3 +202,46 @@ def test_deterministic_and_chunk_independent(gsk, meshgen, tmp_path):
     for r in runs[1:]:
         assert np.array_equal(r["moments"], runs[0]["moments"])
         assert np.array_equal(r["f"], runs[0]["f"])
+
+
+def _moment_errors(a, b):
+    """Per-column max error of moments a vs b: rho, u, T, p relative to the column's largest magnitude;
+    the stress deviator (6 columns) and heat flux (3) relative to the largest pressure, the size of the
+    tensors they belong to (they are themselves small differences of it)."""
+    p_scale = np.max(np.abs(b[:, 5]))
+    out = [rel_err(a[:, k], b[:, k]) for k in range(6)]
+    out += [float(np.max(np.abs(a[:, k] - b[:, k])) / p_scale) for k in range(6, a.shape[1])]
+    return out
+
+
+def test_mfma_moment_sums_track_fp64_sums(gsk, meshgen, tmp_path):
+    """FP32 form: the matrix-core moment contraction (FP32 products and accumulation, FP64 block
+    sums) against the FP64 sums of the same FP32 f, on 24^3 velocities. Stated errors: rho, u, T, p
+    within 2e-6 of their scale, stress deviator and heat flux within 2e-6 of the pressure."""
+    print("[gpu] mfma moment sums", flush=True)
+    path = box_case(meshgen, str(tmp_path), n=(4, 3, 2), n_vel=24, vrange=6.0, kinetic={"precision": "fp32"})
+    ref = gsk.Backend(gsk.Setup(path))
+    mma = gsk.Backend(gsk.Setup(path, {"moment_sums": "mfma"}))
+    f = perturbed_distribution(ref.setup, seed=5).astype(np.float32).astype(np.float64)
+    errs = []
+    for _ in range(2):  # first about zero shifts, then about each cell's velocity
+        ref.set_distribution(f)
+        mma.set_distribution(f)
+        e = _moment_errors(mma.compute_moments(), ref.compute_moments())
+        errs.append(e)
+        print("  per-column errors", ["%.1e" % x for x in e], flush=True)
+    for e in errs:
+        assert max(e) <= 2e-6, e
+
+
+def test_mfma_moment_form_tracks_fp32_oracle(gsk, meshgen, oracle_mod, tmp_path):
+    print("[gpu] mfma moment form vs FP32 oracle", flush=True)
+    path = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp32", "moment_sums": "mfma"})
+    setup = gsk.Setup(path)
+    cpu_setup = gsk.Setup(path, {"moment_sums": "fp64"})
+    cb = oracle_mod.CpuBackend(cpu_setup, "oracle", ops_type=gsk.BackendOps)
+    cpu = _run_solver(gsk, cpu_setup, cb.ops, 4)
+    gpu = _run_solver(gsk, setup, None, 4)
+    assert rel_err(gpu["moments"], cpu["moments"]) <= 1e-4
+    assert rel_err(gpu["f"], cpu["f"]) <= 1e-4
+
