@@ -115,7 +115,10 @@ EXPORTS = [
     "gsk_solver_initialize_equilibrium", "gsk_solver_prepare", "gsk_solver_update_mass_flow_pressure",
     "gsk_solver_sweep", "gsk_solver_patch_mass_flow", "gsk_solver_moments", "gsk_solver_boundary_fluxes",
     "gsk_solver_apply_gsis_correction", "gsk_solver_distribution", "gsk_solver_set_distribution",
-    "gsk_solver_residual_per_cell", "gsk_solver_backend",
+    "gsk_solver_residual_per_cell", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
+    "gsk_solver_set_distribution_device", "gsk_solver_residual_device", "gsk_solver_precondition_device",
+    "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
+    "gsk_solver_set_boundary_iterations",
 ]
 
 _lib_handle = None
@@ -391,6 +394,41 @@ class Solver:
         out = np.empty(self.setup.n_cells, np.float64)
         _check(lib().gsk_solver_residual_per_cell(self._h, _dptr(out), C.c_size_t(out.size)))
         return out
+
+    # ---- device-resident operators (device pointers, n = cells x velocities values of element_size) ----
+    def element_size(self) -> int:
+        out = C.c_int32()
+        _check(lib().gsk_solver_element_size(self._h, C.byref(out)))
+        return out.value
+
+    def distribution_device(self, ptr: int, n: int) -> None:
+        _check(lib().gsk_solver_distribution_device(self._h, C.c_void_p(ptr), C.c_size_t(n)))
+
+    def set_distribution_device(self, ptr: int, n: int) -> None:
+        _check(lib().gsk_solver_set_distribution_device(self._h, C.c_void_p(ptr), C.c_size_t(n)))
+
+    def residual_device(self, ptr: int, n: int) -> None:
+        _check(lib().gsk_solver_residual_device(self._h, C.c_void_p(ptr), C.c_size_t(n)))
+
+    def precondition_device(self, r_ptr: int, out_ptr: int, n: int) -> None:
+        _check(lib().gsk_solver_precondition_device(self._h, C.c_void_p(r_ptr), C.c_void_p(out_ptr), C.c_size_t(n)))
+
+    def mu0(self) -> float:
+        out = C.c_double()
+        _check(lib().gsk_solver_mu0(self._h, C.byref(out)))
+        return out.value
+
+    def set_mu0(self, mu0: float) -> None:
+        _check(lib().gsk_solver_set_mu0(self._h, C.c_double(mu0)))
+
+    def set_accommodation(self, patch: int, alpha: float) -> None:
+        _check(lib().gsk_solver_set_accommodation(self._h, C.c_int32(patch), C.c_double(alpha)))
+
+    def set_newton_tolerances(self, cell_tol: float, inflow_tol: float) -> None:
+        _check(lib().gsk_solver_set_newton_tolerances(self._h, C.c_double(cell_tol), C.c_double(inflow_tol)))
+
+    def set_boundary_iterations(self, n: int) -> None:
+        _check(lib().gsk_solver_set_boundary_iterations(self._h, C.c_int64(n)))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and _lib_handle is not None:

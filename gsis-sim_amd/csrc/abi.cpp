@@ -414,6 +414,83 @@ int gsk_solver_residual_per_cell(gsk_solver* sv, double* out, size_t n) {
   });
 }
 
+int gsk_solver_element_size(gsk_solver* sv, int32_t* out) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(out, "out");
+    *out = sv->sv->element_size();
+  });
+}
+
+int gsk_solver_distribution_device(gsk_solver* sv, void* out, size_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(out, "out");
+    sv->sv->distribution_device(out, n);
+  });
+}
+
+int gsk_solver_set_distribution_device(gsk_solver* sv, const void* in, size_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(in, "in");
+    sv->sv->set_distribution_device(in, n);
+  });
+}
+
+int gsk_solver_residual_device(gsk_solver* sv, void* out, size_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(out, "out");
+    sv->sv->residual_device(out, n);
+  });
+}
+
+int gsk_solver_precondition_device(gsk_solver* sv, const void* r, void* out, size_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(r, "r");
+    need(out, "out");
+    sv->sv->precondition_device(r, out, n);
+  });
+}
+
+int gsk_solver_mu0(gsk_solver* sv, double* out) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(out, "out");
+    *out = sv->sv->mu0();
+  });
+}
+
+int gsk_solver_set_mu0(gsk_solver* sv, double mu0) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->set_mu0(mu0);
+  });
+}
+
+int gsk_solver_set_accommodation(gsk_solver* sv, int32_t patch, double alpha) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->set_accommodation(patch, alpha);
+  });
+}
+
+int gsk_solver_set_newton_tolerances(gsk_solver* sv, double cell_tol, double inflow_tol) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->set_newton_tolerances(cell_tol, inflow_tol);
+  });
+}
+
+int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->set_boundary_iterations(n);
+  });
+}
+
 gsk_backend* gsk_solver_backend(gsk_solver* sv) {
   if (!sv) return nullptr;
   sv->handle.b = &sv->sv->backend();

@@ -4,6 +4,8 @@
 
 #include <cstddef>
 #include <memory>
+#include <stdexcept>
+#include <string>
 
 #include "../../../include/gsis_kinetic.h"
 
@@ -43,6 +45,24 @@ class Backend {
     (void)states;
     (void)n;
     return false;
+  }
+
+  // Device-resident operators for sensitivity solves (no counterpart in the reference backend).
+  // Arrays are device pointers to [cell][storage position] values of the backend's precision.
+  virtual int element_size() const { return 0; }  // sizeof(f); 0 when the operators are absent
+  virtual void residual_field_device(void* out) { unsupported(); }  // steady residual R(f), all velocities
+  virtual void precondition_device(const void* r, void* out) { unsupported(); }  // LU-SGS passes on r, f untouched
+  virtual void distribution_device(void* out) { unsupported(); }
+  virtual void set_distribution_device(const void* in) { unsupported(); }
+  virtual void set_mu0(double mu0) { unsupported(); }                               // viscosity scale
+  // discrete-equilibrium Newton tolerances of cells and of inflow states (defaults 1e-11 / 1e-6 by
+  // precision, and 1e-10)
+  virtual void set_newton_tolerances(double cell_tol, double inflow_tol) { unsupported(); }
+  virtual void set_specular_fraction(const double* frac, std::size_t n) { unsupported(); }  // 1 - alpha per face
+
+ private:
+  [[noreturn]] void unsupported() const {
+    throw std::runtime_error(std::string("backend \"") + name() + "\" has no device-resident sensitivity operators");
   }
 };
 

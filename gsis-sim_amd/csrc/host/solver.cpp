@@ -65,6 +65,8 @@ int discrete_maxwell_host(const KineticSetup& s, double tol, double rho, double 
 
 KineticSolver::KineticSolver(const KineticSetup& setup, std::unique_ptr<Backend> backend)
     : s_(setup), patches_(setup.patches), backend_(std::move(backend)) {
+  mu0_ = setup.mu0;
+  boundary_iterations_ = setup.params.boundary_iterations;
   const auto nc = static_cast<std::size_t>(s_.n_cells());
   const auto nb = static_cast<std::size_t>(s_.n_bfaces());
   moments_.assign(nc * 15, 0.0);
@@ -90,7 +92,7 @@ void KineticSolver::correct_inflow(gsk_inflow_state& in) const {
   g.criterion = s_.deq_criterion;
   if (dev::deq_resolved(g, in.ux, in.uy, in.uz, in.T)) return;
   dev::Maxw<double> m;
-  const int r = discrete_maxwell_host(s_, 1e-10, in.rho, in.ux, in.uy, in.uz, in.T, m);
+  const int r = discrete_maxwell_host(s_, inflow_tol_, in.rho, in.ux, in.uy, in.uz, in.T, m);
   if (r > 0) {
     // maxwell_params_to_state (kinetic_kernels.hpp:543-552)
     in.T = 1.0 / (2.0 * m.B);
@@ -422,7 +424,7 @@ const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
   backend_->boundary_sums(bsums_.data(), bsums_.size());
   process_boundary_sums();
   backend_->set_inflow(inflow_.data(), inflow_.size());
-  for (long long b = 1; b < s_.params.boundary_iterations; ++b) {
+  for (long long b = 1; b < boundary_iterations_; ++b) {
     update_pressure_inflow();
     backend_->set_inflow(inflow_.data(), inflow_.size());
     backend_->boundary_sums(bsums_.data(), bsums_.size());
@@ -611,6 +613,69 @@ void KineticSolver::set_distribution(const double* f, std::size_t n) {
   }
   backend_->set_distribution(internal.data(), internal.size());
   reset_after_initialization();
+}
+
+void KineticSolver::distribution_device(void* out, std::size_t n) {
+  GSK_CHECK(n == static_cast<std::size_t>(s_.n_cells()) * static_cast<std::size_t>(s_.nv()),
+            "distribution length differs from cells x velocities");
+  backend_->distribution_device(out);
+}
+
+void KineticSolver::set_distribution_device(const void* in, std::size_t n) {
+  GSK_CHECK(n == static_cast<std::size_t>(s_.n_cells()) * static_cast<std::size_t>(s_.nv()),
+            "distribution length differs from cells x velocities");
+  backend_->set_distribution_device(in);
+  ++version_;
+}
+
+void KineticSolver::residual_device(void* out, std::size_t n) {
+  GSK_CHECK(n == static_cast<std::size_t>(s_.n_cells()) * static_cast<std::size_t>(s_.nv()),
+            "residual length differs from cells x velocities");
+  ++version_;
+  prepare();
+  backend_->residual_field_device(out);
+}
+
+void KineticSolver::precondition_device(const void* r, void* out, std::size_t n) {
+  GSK_CHECK(n == static_cast<std::size_t>(s_.n_cells()) * static_cast<std::size_t>(s_.nv()),
+            "vector length differs from cells x velocities");
+  prepare();
+  backend_->precondition_device(r, out);
+}
+
+void KineticSolver::set_mu0(double mu0) {
+  GSK_CHECK(mu0 > 0.0, "viscosity scale must be positive");
+  backend_->set_mu0(mu0);
+  mu0_ = mu0;
+  ++version_;
+}
+
+void KineticSolver::set_accommodation(int patch, double alpha) {
+  GSK_CHECK(patch >= 0 && patch < static_cast<int>(patches_.size()), "patch index out of range");
+  PatchCondition& pc = patches_[static_cast<std::size_t>(patch)];
+  GSK_CHECK(pc.kind == kBcMaxwellWall, "accommodation can be changed on Maxwell walls only");
+  GSK_CHECK(alpha > 0.0, "accommodation must be positive");
+  pc.accommodation = alpha;
+  if (specular_.empty()) {
+    specular_.assign(static_cast<std::size_t>(s_.n_bfaces()), 0.0);
+    if (!s_.maxwell_specular.empty()) specular_ = s_.maxwell_specular;
+  }
+  for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) specular_[static_cast<std::size_t>(b)] = 1.0 - alpha;
+  backend_->set_specular_fraction(specular_.data(), specular_.size());
+  ++version_;
+}
+
+void KineticSolver::set_boundary_iterations(long long n) {
+  GSK_CHECK(n >= 1 && n <= 100000, "boundary iterations must be 1..100000");
+  boundary_iterations_ = n;
+  ++version_;
+}
+
+void KineticSolver::set_newton_tolerances(double cell_tol, double inflow_tol) {
+  GSK_CHECK(cell_tol >= 0.0 && inflow_tol >= 0.0, "Newton tolerances must be >= 0");
+  backend_->set_newton_tolerances(cell_tol, inflow_tol);
+  inflow_tol_ = inflow_tol;
+  ++version_;
 }
 
 void KineticSolver::residual_per_cell(double* out, std::size_t n) const {

@@ -36,6 +36,25 @@ class KineticSolver {
   void set_distribution(const double* f, std::size_t n);
   void residual_per_cell(double* out, std::size_t n) const;
 
+  // Device-resident operators for sensitivity solves: arrays are device pointers to
+  // [cell][storage position] values in the f precision (internal numbering). Setting f keeps the
+  // boundary state of the last prepare; the residual prepares for this f first (a function of f
+  // alone once the prepare's inflow / boundary-sum rounds have converged, see
+  // set_boundary_iterations).
+  int element_size() const { return backend_->element_size(); }
+  void distribution_device(void* out, std::size_t n);
+  void set_distribution_device(const void* in, std::size_t n);
+  void residual_device(void* out, std::size_t n);
+  void precondition_device(const void* r, void* out, std::size_t n);
+  double mu0() const { return mu0_; }
+  void set_mu0(double mu0);
+  void set_accommodation(int patch, double alpha);
+  void set_newton_tolerances(double cell_tol, double inflow_tol);
+  // inflow / boundary-sum rounds per prepare ([kinetic].boundary_iterations): the pressure-inlet
+  // reconstruction reads the boundary face states of the previous round, so R(f) is a function of f
+  // alone once these rounds have converged
+  void set_boundary_iterations(long long n);
+
  private:
   void correct_inflow(gsk_inflow_state& in) const;
   void init_static_inflow();
@@ -54,6 +73,10 @@ class KineticSolver {
   std::vector<PatchCondition> patches_;  // mutable copy (mass-flow inlets adjust p)
   std::unique_ptr<Backend> backend_;
   long long iteration_ = 0, version_ = 0, prepared_version_ = -1;
+  double mu0_ = 0.0;
+  double inflow_tol_ = 1e-10;
+  long long boundary_iterations_ = 1;
+  std::vector<double> specular_;  // 1 - alpha per boundary face (Maxwell walls)
   gsk_iteration_info info_{};
   std::vector<double> patch_mass_flow_;
   std::vector<double> moments_, prev_moments_;

@@ -190,11 +190,11 @@ __device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, d
 // Discrete-equilibrium correction of boundary inflow states in FP64, one block per state
 // {scale, rho, ux, uy, uz, T}: a state that is not resolved by the grid is replaced by the
 // Maxwellian whose discrete moments match it (KineticSolver::correct_inflow, kinetic_solver.cpp:103-114;
-// maxwell_params_to_state, kinetic_kernels.hpp:543-552). Newton tolerance 1e-10.
+// maxwell_params_to_state, kinetic_kernels.hpp:543-552). Newton tolerance tol (1e-10 as the reference).
 __global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double* __restrict__ xd,
                                                            const double* __restrict__ yd, const double* __restrict__ zd,
-                                                           double weight, DeqGrid grid, double* __restrict__ states,
-                                                           int n) {
+                                                           double weight, DeqGrid grid, double tol,
+                                                           double* __restrict__ states, int n) {
   __shared__ double lds[kWaves * kDeqSums];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     double* st = states + static_cast<i64>(i) * 6;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double*
     if (!(rho > 0.0) || !(T > 0.0)) continue;
     Maxw<double> m;
     double gamma[5];
-    const int r = block_discrete_maxwell<double>(nv, xd, yd, zd, weight, grid, 1e-10, rho, ux, uy, uz, T, nullptr, m,
+    const int r = block_discrete_maxwell<double>(nv, xd, yd, zd, weight, grid, tol, rho, ux, uy, uz, T, nullptr, m,
                                                  gamma, lds);
     if (r > 0 && threadIdx.x == 0) {
       const double Tn = 1.0 / (2.0 * m.B);
@@ -1467,7 +1467,7 @@ class HipBackend final : public Backend {
     begin(kOther);
     inflow_stage_.upload(reinterpret_cast<const double*>(states), n * 6, stream_);
     dev::k_inflow_correct<<<static_cast<unsigned>(std::min<std::size_t>(n, 65535)), dev::kBlock, 0, stream_>>>(
-        nv_, xd_.p, yd_.p, zd_.p, weight_, view().deq, inflow_stage_.p, static_cast<int>(n));
+        nv_, xd_.p, yd_.p, zd_.p, weight_, view().deq, inflow_tol_, inflow_stage_.p, static_cast<int>(n));
     HIPCHK(hipGetLastError());
     end();
     download(reinterpret_cast<double*>(states), inflow_stage_.p, n * 6);
@@ -1524,66 +1524,138 @@ class HipBackend final : public Backend {
     return static_cast<unsigned>((g + 7) / 8 * 8);
   }
 
+  // gradient and residual R of one velocity chunk into rres_ (and per-cell sums of R^2 into residual_)
+  void chunk_residual(const dev::View<F>& v, const dev::Chunk<F>& ch) {
+    const long long work = 8LL * ((tiles_ + 7) / 8) * nc_;
+    const unsigned grid_irr = capped_grid(static_cast<long long>(n_irregular_) * tiles_);
+    begin(kGradient);
+    for (long long L0 = 0; L0 < work; L0 += grid_cap_)
+      dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0);
+    if (n_irregular_ > 0)
+      dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
+    HIPCHK(hipGetLastError());
+    end();
+    begin(kResidual);
+    for (long long L0 = 0; L0 < work; L0 += grid_cap_)
+      dev::k_residual<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0, rres_.p, partial_.p);
+    if (n_irregular_ > 0)
+      dev::k_residual_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_, rres_.p,
+                                                                   partial_.p);
+    dev::k_residual_reduce<<<(nc_ + 255) / 256, 256, 0, stream_>>>(partial_.p, tiles_, nc_, residual_.p);
+    HIPCHK(hipGetLastError());
+    end();
+  }
+
+  // block_passes x (forward, backward) LU-SGS passes of one chunk on the R in rres_; returns the
+  // buffer holding the final increment, added to f when update_f
+  F* chunk_passes(const dev::View<F>& v, const dev::Chunk<F>& ch, bool update_f) {
+    // velocity tiles of every sweep direction inside this chunk
+    dev::VelRange vr{};
+    int max_tiles = 0;
+    for (int g = 0; g < n_dirs_eff_; ++g) {
+      const int* lo = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g)];
+      const int* hi = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g) + 1];
+      vr.beg[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0) - vel_list_host_.data());
+      vr.end[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0 + ch.kc) - vel_list_host_.data());
+      max_tiles = std::max(max_tiles, (vr.end[g] - vr.beg[g] + 63) / 64);
+    }
+    const int tiles = std::max(max_tiles, 1);
+    const int n_pairs = n_dirs_eff_ * n_blocks_;
+    const long long sgrid_ll = 8LL * ((n_pairs + 7) / 8) * tiles;
+    GSK_CHECK(sgrid_ll * 64 * waves_ < (1LL << 32), "sweep grid too large");
+    const unsigned sgrid = static_cast<unsigned>(sgrid_ll);
+    const dev::SweepArgs fw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, fw_stage_.p,
+                            fw_group_.p, vr};
+    const dev::SweepArgs bw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, bw_stage_.p,
+                            bw_group_.p, vr};
+    F* cur = bufA_.p;
+    F* prev = nullptr;
+    for (int pass = 0; pass < block_passes_; ++pass) {
+      const int has_prev = prev != nullptr ? 1 : 0, last = (update_f && pass + 1 == block_passes_) ? 1 : 0;
+      begin(kForward);
+      dev::k_sweep_forward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
+      HIPCHK(hipGetLastError());
+      end();
+      begin(kBackward);
+      dev::k_sweep_backward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, has_prev, last);
+      HIPCHK(hipGetLastError());
+      end();
+      prev = cur;
+      cur = cur == bufA_.p ? bufB_.p : bufA_.p;
+    }
+    return prev;
+  }
+
   void sweep(double* residual_cell, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
     HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
     const dev::View<F> v = view();
     for (int k0 = 0; k0 < nv_; k0 += chunk_) {
-      dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
-      const long long work = 8LL * ((tiles_ + 7) / 8) * nc_;
-      const unsigned grid_irr = capped_grid(static_cast<long long>(n_irregular_) * tiles_);
-      begin(kGradient);
-      for (long long L0 = 0; L0 < work; L0 += grid_cap_)
-        dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0);
-      if (n_irregular_ > 0)
-        dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
-      HIPCHK(hipGetLastError());
-      end();
-      begin(kResidual);
-      for (long long L0 = 0; L0 < work; L0 += grid_cap_)
-        dev::k_residual<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0, rres_.p, partial_.p);
-      if (n_irregular_ > 0)
-        dev::k_residual_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_,
-                                                                     rres_.p, partial_.p);
-      dev::k_residual_reduce<<<(nc_ + 255) / 256, 256, 0, stream_>>>(partial_.p, tiles_, nc_, residual_.p);
-      HIPCHK(hipGetLastError());
-      end();
-      // velocity tiles of every sweep direction inside this chunk
-      dev::VelRange vr{};
-      int max_tiles = 0;
-      for (int g = 0; g < n_dirs_eff_; ++g) {
-        const int* lo = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g)];
-        const int* hi = vel_list_host_.data() + dir_off_[static_cast<std::size_t>(g) + 1];
-        vr.beg[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0) - vel_list_host_.data());
-        vr.end[g] = static_cast<int>(std::lower_bound(lo, hi, ch.k0 + ch.kc) - vel_list_host_.data());
-        max_tiles = std::max(max_tiles, (vr.end[g] - vr.beg[g] + 63) / 64);
-      }
-      const int tiles = std::max(max_tiles, 1);
-      const int n_pairs = n_dirs_eff_ * n_blocks_;
-      const long long sgrid_ll = 8LL * ((n_pairs + 7) / 8) * tiles;
-      GSK_CHECK(sgrid_ll * 64 * waves_ < (1LL << 32), "sweep grid too large");
-      const unsigned sgrid = static_cast<unsigned>(sgrid_ll);
-      const dev::SweepArgs fw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, fw_stage_.p,
-                              fw_group_.p, vr};
-      const dev::SweepArgs bw{ch.k0, ch.stride, n_blocks_, n_pairs, tiles, block_off_.p, vel_list_.p, bw_stage_.p,
-                              bw_group_.p, vr};
-      F* cur = bufA_.p;
-      F* prev = nullptr;
-      for (int pass = 0; pass < block_passes_; ++pass) {
-        const int has_prev = prev != nullptr ? 1 : 0, update_f = pass + 1 == block_passes_ ? 1 : 0;
-        begin(kForward);
-        dev::k_sweep_forward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
-        HIPCHK(hipGetLastError());
-        end();
-        begin(kBackward);
-        dev::k_sweep_backward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, has_prev, update_f);
-        HIPCHK(hipGetLastError());
-        end();
-        prev = cur;
-        cur = cur == bufA_.p ? bufB_.p : bufA_.p;
-      }
+      const dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
+      chunk_residual(v, ch);
+      chunk_passes(v, ch, true);
     }
     download(residual_cell, residual_.p, n);
+  }
+
+  // ---- device-resident operators for sensitivity solves (arrays [cell][storage position], type F) ----
+  int element_size() const override { return static_cast<int>(sizeof(F)); }
+
+  // copy columns [k0, k0 + kc) between a [nc][nv] array and a [nc][chunk_] workspace
+  void copy_chunk(F* dst, std::size_t dpitch, const F* src, std::size_t spitch, int kc) {
+    HIPCHK(hipMemcpy2DAsync(dst, dpitch * sizeof(F), src, spitch * sizeof(F), static_cast<std::size_t>(kc) * sizeof(F),
+                            static_cast<std::size_t>(nc_), hipMemcpyDeviceToDevice, stream_));
+  }
+
+  void residual_field_device(void* out) override {
+    F* o = static_cast<F*>(out);
+    HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
+    const dev::View<F> v = view();
+    for (int k0 = 0; k0 < nv_; k0 += chunk_) {
+      const dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
+      chunk_residual(v, ch);
+      copy_chunk(o + k0, static_cast<std::size_t>(nv_), rres_.p, static_cast<std::size_t>(chunk_), ch.kc);
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void precondition_device(const void* r, void* out) override {
+    const F* ri = static_cast<const F*>(r);
+    F* o = static_cast<F*>(out);
+    const dev::View<F> v = view();
+    for (int k0 = 0; k0 < nv_; k0 += chunk_) {
+      const dev::Chunk<F> ch{k0, std::min(chunk_, nv_ - k0), chunk_, gx_.p, gy_.p, gz_.p};
+      copy_chunk(rres_.p, static_cast<std::size_t>(chunk_), ri + k0, static_cast<std::size_t>(nv_), ch.kc);
+      F* d = chunk_passes(v, ch, false);
+      copy_chunk(o + k0, static_cast<std::size_t>(nv_), d, static_cast<std::size_t>(chunk_), ch.kc);
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void distribution_device(void* out) override {
+    HIPCHK(hipMemcpyAsync(out, f_.p, f_.bytes(), hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void set_distribution_device(const void* in) override {
+    HIPCHK(hipMemcpyAsync(f_.p, in, f_.bytes(), hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  void set_mu0(double mu0) override { mu0_ = mu0; }
+
+  void set_newton_tolerances(double cell_tol, double inflow_tol) override {
+    deq_.tolerance = cell_tol;
+    inflow_tol_ = inflow_tol;
+  }
+
+  void set_specular_fraction(const double* frac, std::size_t n) override {
+    GSK_CHECK(n == static_cast<std::size_t>(nb_), "specular fractions length differs from the boundary-face count");
+    GSK_CHECK(n_mw_ > 0, "the case has no Maxwell wall");
+    std::vector<F> h(static_cast<std::size_t>(nb_));
+    for (int b = 0; b < nb_; ++b) h[static_cast<std::size_t>(b)] = static_cast<F>(frac[b]);
+    mw_frac_.upload(h.data(), h.size(), stream_);
+    HIPCHK(hipStreamSynchronize(stream_));
   }
 
   long long gsis_correction(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
@@ -1902,6 +1974,7 @@ class HipBackend final : public Backend {
   DBuf<int> irregular_;
   int n_irregular_ = 0;
   bool moments_mfma_ = false;
+  double inflow_tol_ = 1e-10;  // discrete-equilibrium correction of inflow states
   long long grid_cap_ = 8LL << 20;  // workgroups per launch; GSK_GRID_CAP lowers it (test hook)
   DBuf<F> eps2_;
   DBuf<double> volume_, vdt_;

@@ -1,0 +1,293 @@
+"""Sensitivities of the converged inlet mass flow Mdot to the rarefaction parameter delta_rp and to
+a Maxwell wall's accommodation alpha, behind a torch.autograd.Function (no counterpart in the
+reference, which has no derivative capability).
+
+At the converged state f* the steady residual vanishes, R(f*; p) = 0, so
+    dMdot/dp = dMdot/dp|_f + (dMdot/df) s,    J s = -dR/dp,    J = dR/df at f*.
+R is the solver's full steady residual (transport with the limited reconstruction, Shakhov
+collision with the discrete equilibrium, every boundary condition, wall densities and inflow states
+recomputed from f), evaluated on the device through gsk_solver_residual_device. J s = -dR/dp is
+solved by restarted GMRES with right preconditioning by the LU-SGS passes of one inner sweep
+(gsk_solver_precondition_device, the same level-scheduled kernels as the forward solve), with
+Jacobian-vector products as central FP64 differences of R (relative step eps = 1e-5, truncation
+error ~eps^2). The linear system is ill-conditioned (as slow as the CIS iteration itself), so the
+products must be accurate to ~1e-9, and R must be a function of f alone. Two settings make it so
+for the duration of the sensitivity solve (restored afterwards): the discrete-equilibrium Newton
+solves inside R (cells and inflow states) run until the Newton step stops improving instead of
+stopping at a tolerance, and every prepare runs 10 inflow / boundary-sum rounds, because the
+pressure-inlet reconstruction reads the boundary face states of the previous round (one round per
+prepare in the forward solve, where the rounds converge along with the CIS iteration). With these,
+R(f*) repeats to ~1e-9 of a 1e-6 perturbation's response. One linear solve per parameter: for the two
+parameters here this costs what an adjoint solve would.
+
+delta_rp enters only through the nondimensional viscosity scale mu0 = 1 / (sqrt(2) delta_rp); alpha
+only through the specular fraction 1 - alpha of the Maxwell-wall faces.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+
+class SteadyProblem:
+    """A converged forward solve and the device-resident operators around it.
+
+    solver: gsis_sim_amd.Solver (HIP backend); inlet: patch index of the inlet; mdot_factor:
+    Setup.mass_flow_factor(); wall_patch: the Maxwell-wall patch whose accommodation is a
+    parameter (None if alpha is not needed)."""
+
+    def __init__(self, solver, inlet: int, mdot_factor: float, wall_patch: int | None = None,
+                 delta_rp: float | None = None, alpha: float | None = None, eps: float = 1e-5):
+        self.sv = solver
+        self.inlet = inlet
+        self.factor = mdot_factor
+        self.wall_patch = wall_patch
+        self.n = solver.setup.n_cells * solver.setup.nv
+        self.dtype = torch.float64 if solver.element_size() == 8 else torch.float32
+        self.delta_rp = delta_rp if delta_rp is not None else 1.0 / (math.sqrt(2.0) * solver.mu0())
+        self.alpha = alpha
+        self.eps = eps
+        self.boundary_iterations = 1  # the case's [kinetic].boundary_iterations, restored after a solve
+        self.f0 = None
+        self.R0 = None
+        self.mdot0 = None
+        self.stats = {}
+
+    # -- forward --------------------------------------------------------------------------------
+    def set_parameters(self, delta_rp: float | None = None, alpha: float | None = None) -> None:
+        if delta_rp is not None:
+            self.delta_rp = float(delta_rp)
+            self.sv.set_mu0(1.0 / (math.sqrt(2.0) * self.delta_rp))
+        if alpha is not None:
+            self.alpha = float(alpha)
+            self.sv.set_accommodation(self.wall_patch, self.alpha)
+
+    def mdot(self) -> float:
+        return -self.sv.patch_mass_flow()[self.inlet] * self.factor
+
+    def solve(self, residual_target: float = 1e-10, max_iterations: int = 5000, init=None, log=None) -> float:
+        """CIS iterations until the residual norm is below residual_target; returns Mdot."""
+        if init is not None:
+            self.sv.initialize_uniform(*init)
+        t0 = time.perf_counter()
+        it = 0
+        for it in range(max_iterations):
+            self.sv.prepare()
+            info = self.sv.sweep()
+            if log is not None and it % 50 == 0:
+                log(f"iteration {it}: residual {info['residual']:.3e}")
+            if info["residual"] <= residual_target:
+                break
+        self.sv.prepare()
+        self.stats["forward_seconds"] = time.perf_counter() - t0
+        self.stats["forward_iterations"] = it + 1
+        self.f0 = None
+        return self.mdot()
+
+    # -- device operators -----------------------------------------------------------------------
+    def _empty(self) -> torch.Tensor:
+        return torch.empty(self.n, dtype=self.dtype, device="cuda")
+
+    def _set_f(self, f: torch.Tensor) -> None:
+        torch.cuda.synchronize()
+        self.sv.set_distribution_device(f.data_ptr(), self.n)
+
+    def _residual(self, f: torch.Tensor) -> tuple[torch.Tensor, float]:
+        self._set_f(f)
+        out = self._empty()
+        self.sv.residual_device(out.data_ptr(), self.n)
+        return out, self.mdot()
+
+    def _capture(self) -> None:
+        self.f0 = self._empty()
+        torch.cuda.synchronize()
+        self.sv.distribution_device(self.f0.data_ptr(), self.n)
+        self.R0, self.mdot0 = self._residual(self.f0)
+        self.f0_norm = float(torch.linalg.vector_norm(self.f0))
+
+    def _jv(self, z: torch.Tensor) -> tuple[torch.Tensor, float]:
+        """J z and (dMdot/df) z by central differences of R and Mdot at f*."""
+        zn = float(torch.linalg.vector_norm(z))
+        if zn == 0.0:
+            return torch.zeros_like(z), 0.0
+        eps = self.eps * self.f0_norm / zn
+        Rp, mp = self._residual(self.f0 + eps * z)
+        Rm, mm = self._residual(self.f0 - eps * z)
+        return (Rp - Rm) / (2.0 * eps), (mp - mm) / (2.0 * eps)
+
+    def _precondition(self, r: torch.Tensor) -> torch.Tensor:
+        """~ J^-1 r: the LU-SGS passes return -(D + L)^-1 D (D + U)^-1 r."""
+        out = self._empty()
+        torch.cuda.synchronize()
+        self.sv.precondition_device(r.data_ptr(), out.data_ptr(), self.n)
+        return -out
+
+    def _param_derivative(self, which: str) -> tuple[torch.Tensor, float]:
+        """dR/dp and dMdot/dp at fixed f* by a central difference in the parameter."""
+        if which == "delta_rp":
+            mu0 = self.sv.mu0()
+            h = self.eps * mu0
+            self.sv.set_mu0(mu0 + h)
+            Rp, mp = self._residual(self.f0)
+            self.sv.set_mu0(mu0 - h)
+            Rm, mm = self._residual(self.f0)
+            self.sv.set_mu0(mu0)
+            dmu_ddelta = -mu0 / self.delta_rp
+            return (Rp - Rm) / (2.0 * h) * dmu_ddelta, (mp - mm) / (2.0 * h) * dmu_ddelta
+        if which == "alpha":
+            h = self.eps
+            self.sv.set_accommodation(self.wall_patch, self.alpha + h)
+            Rp, mp = self._residual(self.f0)
+            self.sv.set_accommodation(self.wall_patch, self.alpha - h)
+            Rm, mm = self._residual(self.f0)
+            self.sv.set_accommodation(self.wall_patch, self.alpha)
+            return (Rp - Rm) / (2.0 * h), (mp - mm) / (2.0 * h)
+        raise ValueError(which)
+
+    def _gmres(self, b: torch.Tensor, tol: float, restart: int, max_iterations: int):
+        """Right-preconditioned restarted GMRES for J s = b; returns s, iterations, relative residual."""
+        bnorm = float(torch.linalg.vector_norm(b))
+        s = torch.zeros_like(b)
+        if bnorm == 0.0:
+            return s, 0, 0.0
+        total = 0
+        rel = 1.0
+        r = b.clone()
+        while total < max_iterations:
+            beta = float(torch.linalg.vector_norm(r))
+            rel = beta / bnorm
+            if rel <= tol:
+                break
+            m = min(restart, max_iterations - total)
+            V = torch.empty((m + 1, self.n), dtype=self.dtype, device="cuda")
+            V[0] = r / beta
+            H = np.zeros((m + 1, m))
+            cs, sn = np.zeros(m), np.zeros(m)
+            g = np.zeros(m + 1)
+            g[0] = beta
+            k_used = 0
+            for k in range(m):
+                w, _ = self._jv(self._precondition(V[k]))
+                for _ in range(2):  # classical Gram-Schmidt, twice
+                    h = V[: k + 1] @ w
+                    w = w - h @ V[: k + 1]
+                    H[: k + 1, k] += h.double().cpu().numpy()
+                H[k + 1, k] = float(torch.linalg.vector_norm(w))
+                if H[k + 1, k] > 0.0:
+                    V[k + 1] = w / H[k + 1, k]
+                for i in range(k):  # apply the previous Givens rotations
+                    t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
+                    H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]
+                    H[i, k] = t
+                d = math.hypot(H[k, k], H[k + 1, k])
+                cs[k], sn[k] = (H[k, k] / d, H[k + 1, k] / d) if d > 0.0 else (1.0, 0.0)
+                H[k, k] = d
+                H[k + 1, k] = 0.0
+                g[k + 1] = -sn[k] * g[k]
+                g[k] = cs[k] * g[k]
+                k_used = k + 1
+                total += 1
+                rel = abs(g[k + 1]) / bnorm
+                if rel <= tol:
+                    break
+            y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
+            yt = torch.as_tensor(y, dtype=self.dtype, device="cuda")
+            s = s + self._precondition(yt @ V[:k_used])
+            del V
+            Js, _ = self._jv(s)
+            r = b - Js
+        return s, total, rel
+
+    def _bicgstab(self, b: torch.Tensor, tol: float, max_iterations: int):
+        """Right-preconditioned BiCGStab for J s = b (short recurrences: a few vectors of state,
+        two products with J and two preconditioner passes per iteration)."""
+        bnorm = float(torch.linalg.vector_norm(b))
+        x = torch.zeros_like(b)
+        if bnorm == 0.0:
+            return x, 0, 0.0
+        r = b.clone()
+        rhat = b.clone()
+        rho = alpha = omega = 1.0
+        v = torch.zeros_like(b)
+        p = torch.zeros_like(b)
+        rel = 1.0
+        it = 0
+        for it in range(1, max_iterations + 1):
+            rho_new = float(torch.dot(rhat, r))
+            if rho_new == 0.0:
+                break
+            beta = (rho_new / rho) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+            y = self._precondition(p)
+            v, _ = self._jv(y)
+            alpha = rho_new / float(torch.dot(rhat, v))
+            x = x + alpha * y
+            r = r - alpha * v
+            rel = float(torch.linalg.vector_norm(r)) / bnorm
+            if rel <= tol:
+                break
+            z = self._precondition(r)
+            t, _ = self._jv(z)
+            tt = float(torch.dot(t, t))
+            omega = float(torch.dot(t, r)) / tt if tt > 0.0 else 0.0
+            x = x + omega * z
+            r = r - omega * t
+            rel = float(torch.linalg.vector_norm(r)) / bnorm
+            rho = rho_new
+            if rel <= tol or omega == 0.0:
+                break
+        return x, it, rel
+
+    def sensitivities(self, params=("delta_rp", "alpha"), tol: float = 1e-8, restart: int = 30,
+                      max_iterations: int = 400, method: str = "bicgstab") -> dict:
+        """dMdot/dp at the converged state for each named parameter."""
+        t0 = time.perf_counter()
+        self.sv.set_newton_tolerances(0.0, 0.0)
+        self.sv.set_boundary_iterations(10)
+        self._capture()
+        out = {}
+        for p in params:
+            dR, dM = self._param_derivative(p)
+            if method == "gmres":
+                s, its, rel = self._gmres(-dR, tol, restart, max_iterations)
+            else:
+                s, its, rel = self._bicgstab(-dR, tol, max_iterations)
+            _, dMs = self._jv(s)
+            out[p] = dM + dMs
+            self.stats[f"{p}_krylov_iterations"] = its
+            self.stats[f"{p}_krylov_relative_residual"] = rel
+        self.sv.set_newton_tolerances(self.sv.setup.view().deq_tolerance, 1e-10)
+        self.sv.set_boundary_iterations(self.boundary_iterations)
+        self._set_f(self.f0)  # leave the solver at f*
+        self.sv.prepare()
+        self.stats["sensitivity_seconds"] = time.perf_counter() - t0
+        return out
+
+
+class ConvergedMdot(torch.autograd.Function):
+    """Mdot of the converged kinetic solve as a function of (delta_rp, alpha); backward returns the
+    sensitivities from SteadyProblem.sensitivities (FP64 difference Jacobian, LU-SGS-preconditioned
+    GMRES)."""
+
+    @staticmethod
+    def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):
+        problem.set_parameters(float(delta_rp), float(alpha))
+        mdot = problem.solve(residual_target=residual_target)
+        ctx.problem = problem
+        return torch.tensor(mdot, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad):
+        sens = ctx.problem.sensitivities()
+        g = float(grad)
+        return (torch.tensor(g * sens["delta_rp"], dtype=torch.float64),
+                torch.tensor(g * sens["alpha"], dtype=torch.float64), None, None)
+
+
+def converged_mdot(delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem,
+                   residual_target: float = 1e-10) -> torch.Tensor:
+    return ConvergedMdot.apply(delta_rp, alpha, problem, residual_target)

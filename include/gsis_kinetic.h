@@ -240,6 +240,25 @@ int gsk_solver_apply_gsis_correction(gsk_solver* sv, const double* w_half, const
 int gsk_solver_distribution(gsk_solver* sv, double* f, size_t n);
 int gsk_solver_set_distribution(gsk_solver* sv, const double* f, size_t n);
 int gsk_solver_residual_per_cell(gsk_solver* sv, double* out, size_t n);
+
+/* Device-resident operators for sensitivity solves (dMdot/d delta_rp, dMdot/d alpha). No counterpart
+ * in the reference; for callers that keep vectors in HBM (the Python sensitivity driver). Arrays
+ * are device pointers to n = cells x velocities values in the f precision (element_size bytes
+ * each), internal cell numbering and storage velocity order, all on the solver's device. */
+int gsk_solver_element_size(gsk_solver* sv, int32_t* out);                         /* 8 FP64, 4 FP32 */
+int gsk_solver_distribution_device(gsk_solver* sv, void* out, size_t n);           /* f -> out */
+int gsk_solver_set_distribution_device(gsk_solver* sv, const void* in, size_t n);  /* keeps boundary state */
+int gsk_solver_residual_device(gsk_solver* sv, void* out, size_t n);   /* prepare, then steady residual R(f) */
+int gsk_solver_precondition_device(gsk_solver* sv, const void* r, void* out, size_t n); /* LU-SGS on r */
+int gsk_solver_mu0(gsk_solver* sv, double* out);           /* nondimensional viscosity scale */
+int gsk_solver_set_mu0(gsk_solver* sv, double mu0);        /* = 1 / (sqrt(2) delta_rp) for delta_rp cases */
+int gsk_solver_set_accommodation(gsk_solver* sv, int32_t patch, double alpha);  /* Maxwell-wall patch */
+/* discrete-equilibrium Newton tolerances of the cells and of the inflow states (defaults: 1e-11 FP64 /
+ * 1e-6 FP32, and 1e-10); 0 iterates until the Newton step stops improving */
+int gsk_solver_set_newton_tolerances(gsk_solver* sv, double cell_tol, double inflow_tol);
+/* inflow / boundary-sum rounds per prepare ([kinetic].boundary_iterations); the pressure-inlet
+ * reconstruction reads the face states of the previous round */
+int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n);
 gsk_backend* gsk_solver_backend(gsk_solver* sv);
 
 #ifdef __cplusplus
