@@ -191,6 +191,57 @@ def time_to_mdot(g, mg, args):
     print(json.dumps(out), flush=True)
 
 
+def sensitivity_bench(g, mg, args):
+    """Metric 3: cost of dMdot/d delta_rp and dMdot/d alpha (sensitivity solve) relative to one
+    converged forward solve, on the 40k-cell tube with a Maxwell wall (24^3, delta_rp = 10, FP64),
+    with central differences of re-converged solves as the check."""
+    import gsis_sim_amd.sensitivity as sens
+    import torch
+
+    delta, alpha, h = 10.0, args.alpha, args.fd_step
+    tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
+    with tempfile.TemporaryDirectory() as d:
+        path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=delta, accommodation=alpha, kinetic={"precision": "fp64"},
+                            name="tube_maxwell", **tube_args)
+        setup = g.Setup(path)
+        sv = g.Solver(setup)
+        prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
+                                  alpha=alpha)
+        prob.log = lambda msg: log(msg)
+        d_t = torch.tensor(delta, dtype=torch.float64, requires_grad=True)
+        a_t = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
+        sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+        log("forward solve")
+        mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=args.sens_residual)
+        fwd_s, fwd_its = prob.stats["forward_seconds"], prob.stats["forward_iterations"]
+        log(f"forward: {fwd_its} iterations, {fwd_s:.1f} s, Mdot {float(mdot.detach()):.10f}; sensitivity solve")
+        prob.polish = args.polish
+        mdot.backward()
+        out = {"metric": "sensitivity / forward cost", "workload": "tube40k_24cubed_fp64_maxwell",
+               "delta_rp": delta, "alpha": alpha, "mdot": float(mdot.detach()), "forward_seconds": fwd_s,
+               "forward_iterations": fwd_its, "residual_target": args.sens_residual,
+               "sensitivity_seconds": prob.stats["sensitivity_seconds"],
+               "cost_ratio": prob.stats["sensitivity_seconds"] / fwd_s,
+               "dmdot_ddelta": float(d_t.grad), "dmdot_dalpha": float(a_t.grad),
+               "krylov": {k: v for k, v in prob.stats.items() if "krylov" in k}}
+        log(f"sensitivities {out['dmdot_ddelta']:.8e} {out['dmdot_dalpha']:.8e} in {out['sensitivity_seconds']:.1f} s")
+        if args.fd_check:
+            def mdot_at(dl, al):
+                prob.set_parameters(dl, al)
+                m = prob.solve(residual_target=args.fd_residual, log=lambda msg: log(f"fd ({dl}, {al}): {msg}"))
+                log(f"fd ({dl}, {al}): Mdot {m:.12f} after {prob.stats['forward_iterations']} iterations")
+                return m
+            out.update({"fd_step": h, "fd_residual_target": args.fd_residual})
+            if "delta_rp" in args.fd_params:
+                fd_d = (mdot_at(delta * (1 + h), alpha) - mdot_at(delta * (1 - h), alpha)) / (2 * delta * h)
+                out.update({"fd_dmdot_ddelta": fd_d, "rel_err_ddelta": abs(out["dmdot_ddelta"] - fd_d) / abs(fd_d)})
+            if "alpha" in args.fd_params:
+                fd_a = (mdot_at(delta, alpha + h) - mdot_at(delta, alpha - h)) / (2 * h)
+                out.update({"fd_dmdot_dalpha": fd_a, "rel_err_dalpha": abs(out["dmdot_dalpha"] - fd_a) / abs(fd_a)})
+        sv.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +253,17 @@ def main():
                     help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
                          "within 1e-4 of the converged FP64 value")
     ap.add_argument("--max-iterations", type=int, default=800)
+    ap.add_argument("--sensitivity", action="store_true",
+                    help="cost of dMdot/d delta_rp and dMdot/d alpha relative to one converged forward solve "
+                         "(40k tube with a Maxwell wall, FP64)")
+    ap.add_argument("--alpha", type=float, default=0.9, help="Maxwell-wall accommodation for --sensitivity")
+    ap.add_argument("--sens-residual", type=float, default=1e-9, help="forward residual target for --sensitivity")
+    ap.add_argument("--fd-check", action="store_true", help="with --sensitivity: central differences (4 solves)")
+    ap.add_argument("--fd-residual", type=float, default=1e-10, help="residual target of the difference solves")
+    ap.add_argument("--polish", type=int, default=120, help="GMRES iterations polishing the BiCGStab solution")
+    ap.add_argument("--fd-params", default="delta_rp,alpha", help="parameters the differences check")
+    ap.add_argument("--fd-step", type=float, default=1e-3,
+                    help="central-difference step (relative for delta_rp, absolute for alpha)")
     ap.add_argument("--residual-target", type=float, default=1e-6)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 2 GiB (5 times) to calibrate PMC byte counters")
@@ -242,6 +304,9 @@ def main():
 
     if args.time_to_mdot:
         time_to_mdot(g, mg, args)
+        return
+    if args.sensitivity:
+        sensitivity_bench(g, mg, args)
         return
     tube_args, nvel, kinetic = WORKLOADS[args.workload]
     precision = kinetic["precision"]

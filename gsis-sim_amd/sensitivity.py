@@ -52,6 +52,7 @@ class SteadyProblem:
         self.alpha = alpha
         self.eps = eps
         self.boundary_iterations = 1  # the case's [kinetic].boundary_iterations, restored after a solve
+        self.log = None  # optional progress callback (message) for long solves
         self.f0 = None
         self.R0 = None
         self.mdot0 = None
@@ -71,6 +72,7 @@ class SteadyProblem:
 
     def solve(self, residual_target: float = 1e-10, max_iterations: int = 5000, init=None, log=None) -> float:
         """CIS iterations until the residual norm is below residual_target; returns Mdot."""
+        log = log or self.log
         if init is not None:
             self.sv.initialize_uniform(*init)
         t0 = time.perf_counter()
@@ -78,7 +80,7 @@ class SteadyProblem:
         for it in range(max_iterations):
             self.sv.prepare()
             info = self.sv.sweep()
-            if log is not None and it % 50 == 0:
+            if log is not None and it % 25 == 0:
                 log(f"iteration {it}: residual {info['residual']:.3e}")
             if info["residual"] <= residual_target:
                 break
@@ -192,6 +194,8 @@ class SteadyProblem:
                 k_used = k + 1
                 total += 1
                 rel = abs(g[k + 1]) / bnorm
+                if self.log is not None and total % 10 == 0:
+                    self.log(f"GMRES iteration {total}: relative residual {rel:.2e}")
                 if rel <= tol:
                     break
             y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
@@ -238,12 +242,14 @@ class SteadyProblem:
             r = r - omega * t
             rel = float(torch.linalg.vector_norm(r)) / bnorm
             rho = rho_new
+            if self.log is not None and it % 10 == 0:
+                self.log(f"BiCGStab iteration {it}: relative residual {rel:.2e}")
             if rel <= tol or omega == 0.0:
                 break
         return x, it, rel
 
     def sensitivities(self, params=("delta_rp", "alpha"), tol: float = 1e-8, restart: int = 30,
-                      max_iterations: int = 400, method: str = "bicgstab") -> dict:
+                      max_iterations: int = 400, method: str = "bicgstab", polish: int = 0) -> dict:
         """dMdot/dp at the converged state for each named parameter."""
         t0 = time.perf_counter()
         self.sv.set_newton_tolerances(0.0, 0.0)
@@ -252,14 +258,28 @@ class SteadyProblem:
         out = {}
         for p in params:
             dR, dM = self._param_derivative(p)
+            b = -dR
+            bn = float(torch.linalg.vector_norm(b))
             if method == "gmres":
-                s, its, rel = self._gmres(-dR, tol, restart, max_iterations)
+                s, its, rel = self._gmres(b, tol, restart, max_iterations)
             else:
-                s, its, rel = self._bicgstab(-dR, tol, max_iterations)
+                s, its, rel = self._bicgstab(b, tol, max_iterations)
+            # BiCGStab's recurrence residual drifts from the true one b - J s on an ill-conditioned
+            # system: polish with restarted GMRES on the true residual (its residual cannot grow)
+            r_true = b - self._jv(s)[0]
+            true_rel = float(torch.linalg.vector_norm(r_true)) / bn
+            if method != "gmres" and polish > 0 and true_rel > tol:
+                d, k, _ = self._gmres(r_true, tol * bn / float(torch.linalg.vector_norm(r_true)), restart, polish)
+                s = s + d
+                its += k
+                true_rel = float(torch.linalg.vector_norm(b - self._jv(s)[0])) / bn
+            if self.log is not None:
+                self.log(f"{p}: {its} Krylov iterations, true relative residual {true_rel:.2e}")
             _, dMs = self._jv(s)
             out[p] = dM + dMs
             self.stats[f"{p}_krylov_iterations"] = its
             self.stats[f"{p}_krylov_relative_residual"] = rel
+            self.stats[f"{p}_true_relative_residual"] = true_rel
         self.sv.set_newton_tolerances(self.sv.setup.view().deq_tolerance, 1e-10)
         self.sv.set_boundary_iterations(self.boundary_iterations)
         self._set_f(self.f0)  # leave the solver at f*
@@ -282,7 +302,7 @@ class ConvergedMdot(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad):
-        sens = ctx.problem.sensitivities()
+        sens = ctx.problem.sensitivities(polish=getattr(ctx.problem, "polish", 0))
         g = float(grad)
         return (torch.tensor(g * sens["delta_rp"], dtype=torch.float64),
                 torch.tensor(g * sens["alpha"], dtype=torch.float64), None, None)

@@ -25,7 +25,7 @@ def test_sensitivities_match_central_differences(gsk, meshgen, tmp_path):
     mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=1e-12)
     mdot.backward()
     g_delta, g_alpha = float(d_t.grad), float(a_t.grad)
-    print(f"  Mdot {float(mdot):.10f} dMdot/ddelta {g_delta:.8e} dMdot/dalpha {g_alpha:.8e} {prob.stats}", flush=True)
+    print(f"  Mdot {float(mdot.detach()):.10f} dMdot/ddelta {g_delta:.8e} dMdot/dalpha {g_alpha:.8e} {prob.stats}", flush=True)
 
     def mdot_at(dl, al):
         prob.set_parameters(dl, al)
