@@ -517,28 +517,35 @@ __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int*
 // all cells of one tile first, with 256- or 64-velocity tiles, reads rows at the stride of a full
 // velocity row and measured slower.) Grid: 8 * ceil(tiles / 8) * n_cells workgroups.
 // L is the work-item index, L0 + blockIdx.x for a launch that starts at L0 (a multiple of 8, so
-// L mod 8 is the launching XCD); a large chunk is covered by several launches.
-__device__ __forceinline__ bool xcd_tile(i64 L, int n_cells, int tiles, int& cell, int& tile) {
-  const int x = static_cast<int>(L % 8);
-  const i64 k = L / 8;
-  const int ntx = (tiles - x + 7) / 8;  // tiles owned by this XCD
+// L mod 8 is the launching XCD); a large chunk is covered by several launches. The host keeps the
+// total below 2^32 work items, so the index arithmetic is 32-bit unsigned (a 64-bit division
+// costs each wave ~150 scalar instructions, a quarter of the gradient's whole instruction stream).
+__device__ __forceinline__ bool xcd_tile(unsigned L, int n_cells, int tiles, int& cell, int& tile) {
+  const unsigned x = L & 7u;
+  const unsigned k = L >> 3;
+  const int ntx = (tiles - static_cast<int>(x) + 7) / 8;  // tiles owned by this XCD
   if (ntx <= 0) return false;
-  const i64 c = k / ntx;
-  if (c >= n_cells) return false;
+  const unsigned c = k / static_cast<unsigned>(ntx);
+  if (c >= static_cast<unsigned>(n_cells)) return false;
   cell = static_cast<int>(c);
-  tile = x + 8 * static_cast<int>(k % ntx);
+  tile = static_cast<int>(x + 8u * (k - c * static_cast<unsigned>(ntx)));
   return true;
 }
 
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles, i64 L0) {
+__global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles, unsigned L0) {
   int c, tile;
   if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   if (t >= ch.kc) return;
   const int p = ch.k0 + t;
   F gx, gy, gz;
-  limited_gradient_ell(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
+  const EllRow<F>& r = v.ell[c];
+  const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p], fi = f_at(v, c, p);
+  if ((r.skip | r.bnd) == 0)
+    limited_gradient_row<true>(v, r, c, p, x, y, z, fi, gx, gy, gz);
+  else
+    limited_gradient_row<false>(v, r, c, p, x, y, z, fi, gx, gy, gz);
   const i64 wi = static_cast<i64>(c) * ch.stride + t;
   ch.gx[wi] = gx;
   ch.gy[wi] = gy;
@@ -565,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient_csr(View<F> v, Chunk<F> ch,
 
 // residual R into rres and per (cell, tile) partial sums of R^2 (k_residual)
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int tiles, i64 L0, F* __restrict__ rres,
+__global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int tiles, unsigned L0, F* __restrict__ rres,
                                                      double* __restrict__ partial) {
   __shared__ double lds[kWaves];
   int c, tile;
@@ -573,7 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int
   const int t = tile * kBlock + threadIdx.x;
   double r2[1] = {0.0};
   if (t < ch.kc) {
-    const F R = residual_value_ell(v, ch, c, t);
+    const EllRow<F>& r = v.ell[c];
+    const F R = (r.skip | r.bnd) == 0 ? residual_value_row<true>(v, ch, r, c, t) : residual_value_row<false>(v, ch, r, c, t);
     rres[static_cast<i64>(c) * ch.stride + t] = R;
     r2[0] = static_cast<double>(R) * static_cast<double>(R);
   }
@@ -1206,31 +1214,6 @@ class HipBackend final : public Backend {
                 "CSR entry refers to a cell or face out of range");
     }
     csr_.upload(csr.data(), csr.size(), stream_);
-    // ELL copy: kGather entries per cell, padded with entries on the no-op periodic face nb_ (appended
-    // to the boundary-face table below); cells with more faces are listed for the CSR path
-    {
-      dev::CsrE<F> pad{};
-      pad.other = -1;
-      pad.bface = nb_;
-      std::vector<dev::CsrE<F>> ell(static_cast<std::size_t>(nc_) * dev::kGather, pad);
-      std::vector<int> irregular;
-      // test hook: GSK_ELL_MAX_FACES lowers the face count above which a cell takes the CSR path
-      int ell_max = dev::kGather;
-      if (const char* env = std::getenv("GSK_ELL_MAX_FACES")) ell_max = std::max(0, std::min(dev::kGather, std::atoi(env)));
-      if (const char* env = std::getenv("GSK_GRID_CAP")) grid_cap_ = std::max(8LL, std::min(grid_cap_, std::atoll(env)) / 8 * 8);
-      for (int c = 0; c < nc_; ++c) {
-        const int e0 = s.csr_offsets[c], e1 = s.csr_offsets[c + 1];
-        if (e1 - e0 > ell_max) {
-          irregular.push_back(c);
-          continue;
-        }
-        for (int k = e0; k < e1 && k - e0 < dev::kGather; ++k)
-          ell[static_cast<std::size_t>(c) * dev::kGather + static_cast<std::size_t>(k - e0)] = csr[static_cast<std::size_t>(k)];
-      }
-      ell_.upload(ell.data(), ell.size(), stream_);
-      n_irregular_ = static_cast<int>(irregular.size());
-      irregular_.upload(irregular.data(), irregular.size(), stream_);
-    }
     std::vector<F> eps2(static_cast<std::size_t>(nc_));
     for (int c = 0; c < nc_; ++c) eps2[static_cast<std::size_t>(c)] = static_cast<F>(s.eps2[c]);
     eps2_.upload(eps2.data(), eps2.size(), stream_);
@@ -1331,10 +1314,59 @@ class HipBackend final : public Backend {
       HIPCHK(hipMemGetInfo(&free_b, &total_b));
       const double budget = 0.85 * static_cast<double>(free_b);
       const double per_k = 6.0 * nc_ * sizeof(F);
-      while (k > orbit && per_k * k > budget) k = std::max<long long>(orbit, (k / 2 + orbit - 1) / orbit * orbit);
+      while (k > orbit && (per_k * k > budget || work_items(k) > kMaxWorkItems))
+        k = std::max<long long>(orbit, (k / 2 + orbit - 1) / orbit * orbit);
     }
+    if (work_items(k) > kMaxWorkItems)
+      throw std::runtime_error("gradient / residual work items (" + std::to_string(work_items(k)) +
+                               ") exceed 2^32 - 1: use a narrower [kinetic].velocity_chunk");
     chunk_ = static_cast<int>(k);
     tiles_ = (chunk_ + dev::kBlock - 1) / dev::kBlock;
+    // padded face rows (kGather faces per cell; padding faces are skipped like periodic ones); cells
+    // with more faces are listed for the CSR path
+    {
+      std::vector<dev::EllRow<F>> ell(static_cast<std::size_t>(nc_));
+      std::vector<int> irregular;
+      // test hook: GSK_ELL_MAX_FACES lowers the face count above which a cell takes the CSR path
+      int ell_max = dev::kGather;
+      if (const char* env = std::getenv("GSK_ELL_MAX_FACES")) ell_max = std::max(0, std::min(dev::kGather, std::atoi(env)));
+      if (const char* env = std::getenv("GSK_GRID_CAP")) grid_cap_ = std::max(8LL, std::min(grid_cap_, std::atoll(env)) / 8 * 8);
+      for (int c = 0; c < nc_; ++c) {
+        const int e0 = s.csr_offsets[c], e1 = s.csr_offsets[c + 1];
+        const bool csr_path = e1 - e0 > ell_max;  // row left all padding; the CSR-path kernels overwrite it
+        if (csr_path) irregular.push_back(c);
+        dev::EllRow<F>& r = ell[static_cast<std::size_t>(c)];
+        r.skip = r.bnd = 0;
+        for (int j = 0; j < dev::kGather; ++j) {
+          const int k = e0 + j;
+          if (csr_path || k >= e1) {  // padding: the cell's own rows, zero coefficients, never read
+            r.nbf[j] = static_cast<long long>(c) * nv_;
+            r.nbw[j] = static_cast<long long>(c) * chunk_;
+            r.bface[j] = -1;
+            r.skip |= 1u << j;
+            r.sx[j] = r.sy[j] = r.sz[j] = r.dsx[j] = r.dsy[j] = r.dsz[j] = F(0);
+            r.dox[j] = r.doy[j] = r.doz[j] = r.cx[j] = r.cy[j] = r.cz[j] = F(0);
+            continue;
+          }
+          const dev::CsrE<F>& e = csr[static_cast<std::size_t>(k)];
+          const int src = e.other >= 0 ? e.other : c;
+          r.nbf[j] = static_cast<long long>(src) * nv_;
+          r.nbw[j] = static_cast<long long>(src) * chunk_;
+          r.bface[j] = e.other >= 0 ? -1 : e.bface;
+          if (e.other < 0) {
+            r.bnd |= 1u << j;
+            if (s.bfaces[e.bface].kind == dev::kBcPeriodic) r.skip |= 1u << j;
+          }
+          r.sx[j] = e.sx, r.sy[j] = e.sy, r.sz[j] = e.sz;
+          r.dsx[j] = e.dsx, r.dsy[j] = e.dsy, r.dsz[j] = e.dsz;
+          r.dox[j] = e.dox, r.doy[j] = e.doy, r.doz[j] = e.doz;
+          r.cx[j] = e.cx, r.cy[j] = e.cy, r.cz[j] = e.cz;
+        }
+      }
+      ell_.upload(ell.data(), ell.size(), stream_);
+      n_irregular_ = static_cast<int>(irregular.size());
+      irregular_.upload(irregular.data(), irregular.size(), stream_);
+    }
     const std::size_t work = static_cast<std::size_t>(nc_) * static_cast<std::size_t>(chunk_);
     gx_.alloc(work);
     gy_.alloc(work);
@@ -1365,7 +1397,7 @@ class HipBackend final : public Backend {
   int chunk_width() const override { return chunk_; }
   std::size_t memory_bytes() const override {
     return f_.bytes() + gx_.bytes() + gy_.bytes() + gz_.bytes() + rres_.bytes() + bufA_.bytes() + bufB_.bytes() +
-           csr_.bytes() + moments_.bytes() + specular_.bytes() + specular_first_.bytes() + partial_.bytes() +
+           csr_.bytes() + ell_.bytes() + moments_.bytes() + specular_.bytes() + specular_first_.bytes() + partial_.bytes() +
            cells_.bytes() + fw_recs_.bytes() + bw_recs_.bytes() + bsum_part_.bytes() +
            (n_sweep_dirs_ > 0 ? sweep_order_.bytes() + sweep_pos_.bytes() : 0);
   }
@@ -1516,6 +1548,14 @@ class HipBackend final : public Backend {
     dev::k_boundary_reduce<<<(nj + 255) / 256, 256, 0, stream_>>>(faces, n_faces, bsum_chunks_, bsum_part_.p, bsums_.p);
   }
 
+  // gradient / residual work items (workgroups) of a chunk of width k: 8 XCD lanes x ceil(tiles / 8)
+  // x cells; the kernels index them with 32-bit unsigned arithmetic
+  static constexpr long long kMaxWorkItems = 0xFFFFFFFFLL;
+  long long work_items(long long k) const {
+    const long long t = (k + dev::kBlock - 1) / dev::kBlock;
+    return 8LL * ((t + 7) / 8) * nc_;
+  }
+
   // Workgroups of one launch over `work` items: at most grid_cap_ (a multiple of 8, so a kernel's
   // L mod 8 stays its XCD), keeping blocks x threads far below the 2^32 a launch allows; the
   // gradient / residual cover larger work with several launches, the CSR-path kernels loop.
@@ -1526,18 +1566,19 @@ class HipBackend final : public Backend {
 
   // gradient and residual R of one velocity chunk into rres_ (and per-cell sums of R^2 into residual_)
   void chunk_residual(const dev::View<F>& v, const dev::Chunk<F>& ch) {
-    const long long work = 8LL * ((tiles_ + 7) / 8) * nc_;
+    const long long work = work_items(chunk_);
     const unsigned grid_irr = capped_grid(static_cast<long long>(n_irregular_) * tiles_);
     begin(kGradient);
     for (long long L0 = 0; L0 < work; L0 += grid_cap_)
-      dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0);
+      dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, static_cast<unsigned>(L0));
     if (n_irregular_ > 0)
       dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
     HIPCHK(hipGetLastError());
     end();
     begin(kResidual);
     for (long long L0 = 0; L0 < work; L0 += grid_cap_)
-      dev::k_residual<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, L0, rres_.p, partial_.p);
+      dev::k_residual<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, static_cast<unsigned>(L0),
+                                                                               rres_.p, partial_.p);
     if (n_irregular_ > 0)
       dev::k_residual_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_, rres_.p,
                                                                    partial_.p);
@@ -1970,7 +2011,8 @@ class HipBackend final : public Backend {
   DBuf<F> xi_x_, xi_y_, xi_z_;
   DBuf<double> xd_, yd_, zd_;
   DBuf<int> csr_off_;
-  DBuf<dev::CsrE<F>> csr_, ell_;
+  DBuf<dev::CsrE<F>> csr_;
+  DBuf<dev::EllRow<F>> ell_;
   DBuf<int> irregular_;
   int n_irregular_ = 0;
   bool moments_mfma_ = false;

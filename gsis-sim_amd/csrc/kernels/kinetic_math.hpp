@@ -48,6 +48,11 @@ template <class T>
 GSK_HD T dmin(T a, T b) { return b < a ? b : a; }
 template <class T>
 GSK_HD T dmax(T a, T b) { return a < b ? b : a; }
+// the same for operands that are neither NaN nor -0 (one v_max / v_min instead of compare + select)
+GSK_HD double nmax(double a, double b) { return ::fmax(a, b); }
+GSK_HD float nmax(float a, float b) { return ::fmaxf(a, b); }
+GSK_HD double nmin(double a, double b) { return ::fmin(a, b); }
+GSK_HD float nmin(float a, float b) { return ::fminf(a, b); }
 
 // f_eq = A exp(-B |xi - u|^2)
 template <class F>

@@ -19,6 +19,25 @@ struct CsrE {
   F cx, cy, cz;
 };
 
+// One cell's faces for the gradient and residual kernels, padded to kGather faces (ELL form). The
+// integer part comes first and the per-face coefficients are stored field by field, so a wave reads
+// a whole row in a few wide scalar loads and can issue all neighbour loads together; neighbour
+// offsets are precomputed 64-bit element offsets, so no address needs a scalar multiply.
+constexpr int kGather = 6;  // faces per row (hex 6, prism/pyramid 5, tet 4); cells with more: CSR path
+
+template <class F>
+struct EllRow {
+  i64 nbf[kGather];    // neighbour row in f: other * nv (the cell's own row for boundary faces)
+  i64 nbw[kGather];    // neighbour row in the chunk workspace: other * chunk stride (own row likewise)
+  int bface[kGather];  // boundary face index, -1 for internal faces
+  unsigned skip;       // bit k: face k is periodic or padding, a face no operator reads
+  unsigned bnd;        // bit k: face k is a boundary face
+  F sx[kGather], sy[kGather], sz[kGather];
+  F dsx[kGather], dsy[kGather], dsz[kGather];
+  F dox[kGather], doy[kGather], doz[kGather];
+  F cx[kGather], cy[kGather], cz[kGather];
+};
+
 template <class F>
 struct BFace {
   int owner, kind, mirror_xor, patch;
@@ -42,7 +61,7 @@ struct View {
   double weight_d;
   const int* csr_off;
   const CsrE<F>* csr;
-  const CsrE<F>* ell;  // [cell][kGather] copy of the CSR entries, padded with a no-op periodic face
+  const EllRow<F>* ell;  // [cell] padded face rows (cells with more than kGather faces: CSR path)
   const F* eps2;
   int limiter_mode, positivity;
   F diag_scale, interface_damping;
@@ -86,21 +105,51 @@ __device__ __forceinline__ F inflow_value(const Inflow<F>& b, F x, F y, F z) {
 
 // first-order boundary value for gradient / limiter (kinetic_kernels.hpp:145-162)
 template <class F>
-__device__ F boundary_value_first_order(const View<F>& v, const CsrE<F>& e, int cell, int p, F x, F y, F z, F fi) {
-  const BFace<F>& b = v.bf[e.bface];
+__device__ F boundary_value_first_order(const View<F>& v, int bface, F sx, F sy, F sz, int cell, int p, F x, F y, F z,
+                                        F fi) {
+  const BFace<F>& b = v.bf[bface];
   if (b.kind == kBcPeriodic) return fi;
   if (b.kind == kBcSymmetry) return F(0.5) * (fi + f_at(v, cell, p ^ b.mirror_xor));
-  const F a = x * e.sx + y * e.sy + z * e.sz;
+  const F a = x * sx + y * sy + z * sz;
   if (a >= F(0)) return fi;
   if (b.kind == kBcMaxwell)
-    return inflow_value(v.inflow[e.bface], x, y, z) + v.mw_frac[e.bface] * v.specular_first[v.mw_off[e.bface] + p];
-  return inflow_value(v.inflow[e.bface], x, y, z);
+    return inflow_value(v.inflow[bface], x, y, z) + v.mw_frac[bface] * v.specular_first[v.mw_off[bface] + p];
+  return inflow_value(v.inflow[bface], x, y, z);
+}
+
+template <class F>
+__device__ __forceinline__ F boundary_value_first_order(const View<F>& v, const CsrE<F>& e, int cell, int p, F x, F y,
+                                                        F z, F fi) {
+  return boundary_value_first_order(v, e.bface, e.sx, e.sy, e.sz, cell, p, x, y, z, fi);
 }
 
 // limited LSQ gradient phi * grad f at (cell, p) (kinetic_kernels.hpp:165-201).
 // The neighbour values are gathered first (all loads in flight together), then combined in CSR order,
 // so the arithmetic is the reference's while the wave waits on memory once instead of once per face.
-constexpr int kGather = 6;  // faces held in registers (hex 6, prism/pyramid 5, tet 4); more: CSR path
+
+// phi * grad from the largest and the most negative face projection grad . ds (over the faces the
+// limiter reads). The reference takes phi = min over faces of venkat_face(proj_k). For proj > 0
+// (d1 = dmx >= 0) the clamped limiter function is 1 up to proj = dmx / 2 and strictly decreasing
+// beyond (its derivative has the sign of (d1^2 + eps2)(d1 - 4 proj) - 4 d1 proj^2 < 0 there), so it
+// is non-increasing in proj; the mirror statement holds for proj < 0 with d1 = dmn <= 0. The minimum
+// over faces is therefore attained at the two extreme projections: two divisions per (cell,
+// velocity) instead of one per face, with the same value up to rounding.
+template <class F>
+__device__ __forceinline__ void limit_from_extremes(const View<F>& v, int cell, F fi, F fmx, F fmn, F proj_max,
+                                                    F proj_min, F gx, F gy, F gz, F& ogx, F& ogy, F& ogz) {
+  F eps2 = v.eps2[cell];
+  if (v.limiter_mode == kLimRel) {
+    const F s = dmax(dabs(fmx), dabs(fmn));
+    eps2 *= s * s;
+  }
+  const F dmx = fmx - fi, dmn = fmn - fi;
+  F phi = F(1);
+  if (v.limiter_mode != kLimNone) phi = dmin(venkat_face(proj_max, dmx, dmn, eps2), venkat_face(proj_min, dmx, dmn, eps2));
+  if (v.positivity != 0 && proj_min < F(0)) phi = fi > F(0) ? dmin(phi, fi / (-proj_min)) : F(0);
+  ogx = phi * gx;
+  ogy = phi * gy;
+  ogz = phi * gz;
+}
 
 template <class F>
 __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, const CsrE<F>* ents, int n, F fi, F fmx,
@@ -115,25 +164,15 @@ __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, cons
     ogx = ogy = ogz = F(0);
     return;
   }
-  F eps2 = v.eps2[cell];
-  if (v.limiter_mode == kLimRel) {
-    const F s = dmax(dabs(fmx), dabs(fmn));
-    eps2 *= s * s;
-  }
-  const F dmx = fmx - fi, dmn = fmn - fi;
-  F phi = F(1);
-  F proj_min = F(0);
+  F proj_max = F(0), proj_min = F(0);
   for (int k = 0; k < n; ++k) {
     const CsrE<F>& e = ents[k];
     if (e.other < 0 && v.bf[e.bface].kind == kBcPeriodic) continue;
     const F proj = gx * e.dsx + gy * e.dsy + gz * e.dsz;
-    if (v.limiter_mode != kLimNone) phi = dmin(phi, venkat_face(proj, dmx, dmn, eps2));
+    proj_max = dmax(proj_max, proj);
     proj_min = dmin(proj_min, proj);
   }
-  if (v.positivity != 0 && proj_min < F(0)) phi = fi > F(0) ? dmin(phi, fi / (-proj_min)) : F(0);
-  ogx = phi * gx;
-  ogy = phi * gy;
-  ogz = phi * gz;
+  limit_from_extremes(v, cell, fi, fmx, fmn, proj_max, proj_min, gx, gy, gz, ogx, ogy, ogz);
 }
 
 // gradient from at most kGather entries ents[0..n): neighbour values gathered first, then combined
@@ -187,15 +226,54 @@ __device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F 
   limit_and_scale(v, cell, v.csr + e0, e1 - e0, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
 }
 
-// The same gradient from the cell's padded row of the ELL table: the entry count is the constant
-// kGather, so the entry loads issue together with no branch on the face count. The padding entries
-// name a periodic face: first-order value fi (d = 0, adds +0 to sums that start at +0 and so never
-// hold -0; fmx / fmn unchanged), skipped by the limiter. Cells with more faces get a wrong value
-// here and are recomputed by the CSR path afterwards.
-template <class F>
-__device__ __forceinline__ void limited_gradient_ell(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx,
-                                                     F& ogy, F& ogz) {
-  gradient_gathered(v, cell, p, x, y, z, fi, v.ell + static_cast<i64>(cell) * kGather, kGather, ogx, ogy, ogz);
+// The same gradient from the cell's padded row. Periodic and padding faces are skipped: the
+// reference's first-order value there is fi, so the face adds e.c * 0 = +-0 to sums that start at +0
+// (and so never hold -0, leaving them unchanged) and leaves fmx / fmn as they are, and the limiter
+// skips periodic faces. Cells with more faces get a wrong value here and are recomputed by the CSR
+// path afterwards. Interior = true is the form for rows whose faces are all internal (no mask
+// tests, so the row's loads merge and nothing branches per face). Distribution values are never -0
+// (they start >= +0 and a sum is -0 only if both terms are), so nmax / nmin equal dmax / dmin on
+// them; for the projections, +0 and -0 both give the limiter value 1 and fail proj_min < 0.
+template <bool Interior, class F>
+__device__ __forceinline__ void limited_gradient_row(const View<F>& v, const EllRow<F>& r, int cell, int p, F x, F y,
+                                                     F z, F fi, F& ogx, F& ogy, F& ogz) {
+  const F* fp = v.f + p;
+  const unsigned skip = Interior ? 0u : r.skip, bnd = Interior ? 0u : r.bnd;
+  F fk[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) fk[k] = fp[r.nbf[k]];  // boundary and padding faces: the cell's own row
+  F gx = F(0), gy = F(0), gz = F(0);
+  F fmx = fi, fmn = fi;
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) {
+    if ((skip >> k) & 1u) continue;
+    if ((bnd >> k) & 1u) fk[k] = boundary_value_first_order(v, r.bface[k], r.sx[k], r.sy[k], r.sz[k], cell, p, x, y, z, fi);
+    const F d = fk[k] - fi;
+    gx += r.cx[k] * d;
+    gy += r.cy[k] * d;
+    gz += r.cz[k] * d;
+    fmx = nmax(fmx, fk[k]);
+    fmn = nmin(fmn, fk[k]);
+  }
+  if (v.limiter_mode == kLimNone && v.positivity == 0) {
+    ogx = gx;
+    ogy = gy;
+    ogz = gz;
+    return;
+  }
+  if (v.positivity != 0 && !(fi > F(0))) {
+    ogx = ogy = ogz = F(0);
+    return;
+  }
+  F proj_max = F(0), proj_min = F(0);
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) {
+    if ((skip >> k) & 1u) continue;
+    const F proj = gx * r.dsx[k] + gy * r.dsy[k] + gz * r.dsz[k];
+    proj_max = nmax(proj_max, proj);
+    proj_min = nmin(proj_min, proj);
+  }
+  limit_from_extremes(v, cell, fi, fmx, fmn, proj_max, proj_min, gx, gy, gz, ogx, ogy, ogz);
 }
 
 // outgoing reconstructed value at a boundary face (kinetic_kernels.hpp:204-211)
@@ -378,16 +456,81 @@ __device__ F residual_value(const View<F>& v, const Chunk<F>& ch, int cell, int 
   return R;
 }
 
-// The same residual from the cell's padded ELL row (see limited_gradient_ell); padding entries are
-// periodic faces, which the flux sum skips.
-template <class F>
-__device__ F residual_value_ell(const View<F>& v, const Chunk<F>& ch, int cell, int t) {
+// The same residual from the cell's padded row: upwind neighbour values (f and its limited gradient)
+// are loaded by the lanes that need them, all faces at once, then the fluxes are added in face order.
+// Periodic and padding faces are skipped, as the reference skips periodic faces. Interior = true:
+// as in limited_gradient_row.
+template <bool Interior, class F>
+__device__ __forceinline__ F residual_value_row(const View<F>& v, const Chunk<F>& ch, const EllRow<F>& r, int cell,
+                                                int t) {
   const int p = ch.k0 + t;
   const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
-  const F fi = f_at(v, cell, p);
+  const F* fp = v.f + p;
+  const F fi = fp[static_cast<i64>(cell) * v.nv];
   const i64 wi = static_cast<i64>(cell) * ch.stride + t;
   const F gix = ch.gx[wi], giy = ch.gy[wi], giz = ch.gz[wi];
-  F R = flux_gathered(v, ch, cell, t, p, x, y, z, fi, gix, giy, giz, v.ell + static_cast<i64>(cell) * kGather, kGather);
+  const unsigned skip = Interior ? 0u : r.skip, bnd = Interior ? 0u : r.bnd;
+  // ak: face flux coefficient; fup: reconstructed upwind value of an internal face, set only in the
+  // lanes that are downwind of it (ak < 0), the only lanes that read it
+  F ak[kGather], fup[kGather];
+  if constexpr (sizeof(F) == 4) {
+    // FP32: every face's neighbour loads are issued before any is used (more registers; measured
+    // 5% faster than the per-face form below)
+    F fu[kGather], gux[kGather], guy[kGather], guz[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k) {
+      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      fu[k] = gux[k] = guy[k] = guz[k] = F(0);
+      if (((skip | bnd) >> k) & 1u) continue;
+      if (ak[k] < F(0)) {
+        const i64 wj = r.nbw[k] + t;
+        fu[k] = fp[r.nbf[k]];
+        gux[k] = ch.gx[wj];
+        guy[k] = ch.gy[wj];
+        guz[k] = ch.gz[wj];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k) fup[k] = fu[k] + gux[k] * r.dox[k] + guy[k] * r.doy[k] + guz[k] * r.doz[k];
+  } else {
+    // FP64: the face's value is formed where its loads land (58 VGPRs instead of 104, so twice the
+    // waves per SIMD hide the latency; all-loads-first measured 40% slower here)
+#pragma unroll
+    for (int k = 0; k < kGather; ++k) {
+      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      fup[k] = F(0);
+      if (((skip | bnd) >> k) & 1u) continue;
+      if (ak[k] < F(0)) {
+        const i64 wj = r.nbw[k] + t;
+        fup[k] = fp[r.nbf[k]] + ch.gx[wj] * r.dox[k] + ch.gy[wj] * r.doy[k] + ch.gz[wj] * r.doz[k];
+      }
+    }
+  }
+  F R = F(0);
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) {
+    if ((skip >> k) & 1u) continue;
+    const F a = ak[k];
+    F ff;
+    if (a >= F(0)) {
+      ff = fi + gix * r.dsx[k] + giy * r.dsy[k] + giz * r.dsz[k];
+    } else if (!((bnd >> k) & 1u)) {
+      ff = fup[k];
+    } else {
+      const int b = r.bface[k];
+      const int kind = v.bf[b].kind;
+      if (kind == kBcSymmetry) {
+        const int tm = t ^ v.bf[b].mirror_xor;
+        const i64 wm = static_cast<i64>(cell) * ch.stride + tm;
+        ff = f_at(v, cell, ch.k0 + tm) + ch.gx[wm] * r.dsx[k] + ch.gy[wm] * r.dsy[k] + ch.gz[wm] * r.dsz[k];
+      } else if (kind == kBcMaxwell) {
+        ff = maxwell_inflow(v, b, p, x, y, z);
+      } else {
+        ff = inflow_value(v.inflow[b], x, y, z);
+      }
+    }
+    R += a * ff;
+  }
   const CellP<F>& cp = v.cells[cell];
   R -= cp.coll * (target_eval(cp.s, x, y, z) - fi);
   return R;
