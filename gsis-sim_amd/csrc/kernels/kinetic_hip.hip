@@ -634,17 +634,22 @@ struct VelRange {
 // class each has in that sweep, so the kernel needs no order / position / boundary lookups.
 // code = neighbour << 2 | class; class 0 skip (periodic), 1 contributes a+ only (boundary, or an
 // in-block neighbour on the downstream side), 2 in-block upstream neighbour, 3 out-of-block
-// neighbour. nf = -1: more than kGather faces, use the CSR path.
+// neighbour. The record keeps the faces of classes 2, 3 and 1 in that order (each group in CSR
+// order) with their row offsets precomputed for the backend's chunk width, so the kernels test no
+// class and multiply no index. nf = -1: more than kGather faces, use the CSR path.
 // diag is the cell's V/tau + V/dt, copied in by k_record_diag whenever the cell parameters change.
 //
 // The records of a (direction, block) are stored level by level; a level wider than one LDS stage
 // is cut into several stages (its cells are independent, so the cut only adds a barrier).
 template <class F>
 struct SweepRec {
-  int cell, nf;
-  int code[kGather];
+  int cell, nf;        // nf: faces that take part (slots [0, nf)); -1: more than kGather faces, CSR path
+  int n_in, n_out;     // the first n_in slots are in-block upstream faces, the next n_out out-of-block ones
+  i64 row_w;           // byte offset of the cell's row in the chunk buffers (cell * chunk width * sizeof(F))
+  i64 row_f;           // byte offset of the cell's row in f (cell * nv * sizeof(F))
+  i64 nb_w[kGather];   // byte offsets of the neighbours' rows in the chunk buffers (first two groups)
   F diag;
-  F sx[kGather], sy[kGather], sz[kGather];
+  F sx[kGather], sy[kGather], sz[kGather];  // zero in the slots past nf
 };
 template <class F>
 constexpr int kRecWords = static_cast<int>(sizeof(SweepRec<F>) / 4);
@@ -814,95 +819,97 @@ __device__ __forceinline__ void backward_cell(const View<F>& v, int cell, int t,
   if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
 }
 
-// One cell from its sweep record in LDS: same results as forward_cell. The record's integers are
+__device__ __forceinline__ i64 uniform64(i64 x) {
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32)));
+  return static_cast<i64>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <class F>
+__device__ __forceinline__ F* byte_row(F* base, i64 off) {
+  return reinterpret_cast<F*>(reinterpret_cast<char*>(base) + off);
+}
+template <class F>
+__device__ __forceinline__ const F* byte_row(const F* base, i64 off) {
+  return reinterpret_cast<const F*>(reinterpret_cast<const char*>(base) + off);
+}
+
+// One cell from its sweep record in LDS: the results of forward_cell. The record's integers are
 // made wave-uniform (scalar registers), its areas are LDS operands. A neighbour row is loaded for
 // the whole wave when any lane is upwind of that face (one scalar branch, no per-lane masks), and
 // the accumulation selects per lane: adding +0 for a face a lane skips leaves every partial sum
-// unchanged bit for bit (the sums start at +0 and never become -0), so the sums equal the
-// sequential ones. toff = the lane velocity's byte offset in a chunk row.
-template <class F>
-__device__ __forceinline__ void forward_rec(const View<F>& v, const SweepRec<F>& r, int nf, F x, F y, F z,
-                                            unsigned toff, int stride, const F* __restrict__ rres, F* cur,
+// unchanged bit for bit (the sums start at +0 and never become -0). The upwind sums L and Lout run
+// over their faces in CSR order as in the reference; the outflow sum ap runs over the record's
+// grouped order, which can move it by an ulp. Unused slots have zero areas: a = +-0 adds +0 to ap
+// and nothing else. toff = the lane velocity's byte offset in a chunk row. HasPrev: a pass after
+// the first (out-of-block upwind values from the previous pass instead of the damping term).
+template <bool HasPrev, class F>
+__device__ __forceinline__ void forward_rec(const SweepRec<F>& r, F x, F y, F z, F diag_scale, F damping,
+                                            unsigned toff, const F* __restrict__ rres, F* cur,
                                             const F* __restrict__ prev) {
-  const i64 row = static_cast<i64>(uniform(r.cell)) * stride;
-  const F Ri = row_load(rres + row, toff);
+  const int n_in = uniform(r.n_in), n_io = n_in + uniform(r.n_out);
+  const i64 row = uniform64(r.row_w);
+  const F Ri = row_load(byte_row(rres, row), toff);
   F ak[kGather], nb[kGather];
-  int cls[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    cls[k] = 0;
-    ak[k] = F(0);
+    ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
     nb[k] = F(0);
-    if (k < nf) {
-      const int code = uniform(r.code[k]);
-      cls[k] = code & 3;
-      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
-      const F* base = cls[k] == 2 ? cur : (cls[k] == 3 ? prev : nullptr);
-      if (base != nullptr && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
-        nb[k] = row_load(base + static_cast<i64>(code >> 2) * stride, toff);
-    }
+    // one load site per face (in-block rows from this pass, out-of-block rows from the previous one)
+    const bool load = HasPrev ? k < n_io : k < n_in;
+    const F* base = (!HasPrev || k < n_in) ? static_cast<const F*>(cur) : prev;
+    if (load && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
+      nb[k] = row_load(byte_row(base, uniform64(r.nb_w[k])), toff);
   }
   F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    if (cls[k] == 0) continue;
     const F a = ak[k];
     const bool down = a > F(0);
     ap += down ? a : F(0);
-    if (cls[k] == 3) {
-      if (prev != nullptr)
+    if (k < n_in) {
+      L += down ? F(0) : a * nb[k];
+    } else if (k < n_io) {
+      if (HasPrev)
         Lout += down ? F(0) : a * nb[k];
       else
         am_out -= down ? F(0) : a;
-    } else if (cls[k] == 2) {
-      L += down ? F(0) : a * nb[k];
     }
   }
-  const F D = r.diag + v.diag_scale * ap + v.interface_damping * am_out;
-  row_ref(cur + row, toff) = (-Ri - Lout - L) / D;
+  const F D = r.diag + diag_scale * ap + damping * am_out;
+  row_ref(byte_row(cur, row), toff) = (-Ri - Lout - L) / D;
 }
 
-// One cell from its sweep record in LDS: same results as backward_cell (same scheme as
+// One cell from its sweep record in LDS: the results of backward_cell (same scheme as
 // forward_rec). poff = the lane velocity's byte offset in an f row.
-template <class F>
-__device__ __forceinline__ void backward_rec(const View<F>& v, const SweepRec<F>& r, int nf, F x, F y, F z,
-                                             unsigned toff, unsigned poff, int stride, F* cur, int has_prev,
-                                             int update_f) {
-  const int cell = uniform(r.cell);
-  const i64 row = static_cast<i64>(cell) * stride;
-  const F own = row_load(cur + row, toff);
+template <bool HasPrev, class F>
+__device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z, F diag_scale, F damping,
+                                             unsigned toff, unsigned poff, F* cur, F* f, int update_f) {
+  const int n_in = uniform(r.n_in), n_io = n_in + uniform(r.n_out);
+  const i64 row = uniform64(r.row_w);
+  const F own = row_load(byte_row(cur, row), toff);
   F ak[kGather], nb[kGather];
-  int cls[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    cls[k] = 0;
-    ak[k] = F(0);
+    ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
     nb[k] = F(0);
-    if (k < nf) {
-      const int code = uniform(r.code[k]);
-      cls[k] = code & 3;
-      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
-      if (cls[k] == 2 && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
-        nb[k] = row_load(cur + static_cast<i64>(code >> 2) * stride, toff);
-    }
+    if (k < n_in && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
+      nb[k] = row_load(byte_row(cur, uniform64(r.nb_w[k])), toff);
   }
   F ap = F(0), am_out = F(0), U = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    if (cls[k] == 0) continue;
     const F a = ak[k];
     const bool down = a > F(0);
     ap += down ? a : F(0);
-    if (cls[k] == 3) {
-      if (!has_prev) am_out -= down ? F(0) : a;
-    } else if (cls[k] == 2) {
+    if (k < n_in)
       U += down ? F(0) : a * nb[k];
-    }
+    else if (!HasPrev && k < n_io)
+      am_out -= down ? F(0) : a;
   }
-  const F D = r.diag + v.diag_scale * ap + v.interface_damping * am_out;
+  const F D = r.diag + diag_scale * ap + damping * am_out;
   const F df = own - U / D;
-  row_ref(cur + row, toff) = df;
-  if (update_f) row_ref(v.f + static_cast<i64>(cell) * v.nv, poff) += df;
+  row_ref(byte_row(cur, row), toff) = df;
+  if (update_f) row_ref(byte_row(f, uniform64(r.row_f)), poff) += df;
 }
 
 // Workgroup -> (direction, block, velocity tile). The (direction, block) pairs are dealt to the 8
@@ -944,7 +951,7 @@ struct SweepArgs {
 // stage-end barrier. Records come from LDS (not through the vector memory counter), so a cell's
 // only memory round trip is its neighbour gather, which overlaps the previous cell's store. The
 // waves take every nwaves-th cell of a stage; lanes = velocities of one direction.
-template <class F>
+template <class F, bool HasPrev>
 __global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_forward(View<F> v, SweepArgs a, const SweepRec<F>* __restrict__ recs,
                                 const F* __restrict__ rres, F* cur, const F* __restrict__ prev) {
   int g, b, tile;
@@ -975,7 +982,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_forward(View<F> 
       const int nf = uniform(r.nf);
       if (!valid) continue;
       if (nf >= 0)
-        forward_rec(v, r, nf, x, y, z, toff, a.stride, rres, cur, prev);
+        forward_rec<HasPrev>(r, x, y, z, v.diag_scale, v.interface_damping, toff, rres, cur, prev);
       else
         forward_cell(v, uniform(r.cell), t, c0, c1, pos, x, y, z, a.stride, rres, cur, prev);
     }
@@ -983,9 +990,10 @@ __global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_forward(View<F> 
   }
 }
 
-template <class F>
-__global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_backward(View<F> v, SweepArgs a, const SweepRec<F>* __restrict__ recs, F* cur, int has_prev,
+template <class F, bool HasPrev>
+__global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_backward(View<F> v, SweepArgs a, const SweepRec<F>* __restrict__ recs, F* cur,
                                  int update_f) {
+  constexpr int has_prev = HasPrev ? 1 : 0;
   int g, b, tile;
   if (!sweep_tile(a.n_pairs, a.tiles, a.n_blocks, g, b, tile)) return;
   const int vb = a.vr.beg[g] + tile * 64;
@@ -1014,7 +1022,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_backward(View<F>
       const int nf = uniform(r.nf);
       if (!valid) continue;
       if (nf >= 0)
-        backward_rec(v, r, nf, x, y, z, toff, poff, a.stride, cur, has_prev, update_f);
+        backward_rec<HasPrev>(r, x, y, z, v.diag_scale, v.interface_damping, toff, poff, cur, v.f, update_f);
       else
         backward_cell(v, uniform(r.cell), t, p, c0, c1, pos, x, y, z, a.stride, cur, has_prev, update_f);
     }
@@ -1250,7 +1258,6 @@ class HipBackend final : public Backend {
       sweep_pos_.upload(s.sweep_pos, static_cast<std::size_t>(n_sweep_dirs_) * nc_, stream_);
       sweep_dir_of_.upload(s.sweep_dir_of, static_cast<std::size_t>(nv_), stream_);
     }
-    build_levels(s);
     // Maxwell walls
     n_mw_ = s.n_maxwell_faces;
     moments_mfma_ = s.moments_mfma != 0;
@@ -1367,6 +1374,7 @@ class HipBackend final : public Backend {
       n_irregular_ = static_cast<int>(irregular.size());
       irregular_.upload(irregular.data(), irregular.size(), stream_);
     }
+    build_levels(s);  // after chunk_: the sweep records hold row offsets for the chunk width
     const std::size_t work = static_cast<std::size_t>(nc_) * static_cast<std::size_t>(chunk_);
     gx_.alloc(work);
     gy_.alloc(work);
@@ -1614,11 +1622,17 @@ class HipBackend final : public Backend {
     for (int pass = 0; pass < block_passes_; ++pass) {
       const int has_prev = prev != nullptr ? 1 : 0, last = (update_f && pass + 1 == block_passes_) ? 1 : 0;
       begin(kForward);
-      dev::k_sweep_forward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
+      if (has_prev)
+        dev::k_sweep_forward<F, true><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
+      else
+        dev::k_sweep_forward<F, false><<<sgrid, 64 * waves_, 0, stream_>>>(v, fw, fw_recs_.p, rres_.p, cur, prev);
       HIPCHK(hipGetLastError());
       end();
       begin(kBackward);
-      dev::k_sweep_backward<F><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, has_prev, last);
+      if (has_prev)
+        dev::k_sweep_backward<F, true><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, last);
+      else
+        dev::k_sweep_backward<F, false><<<sgrid, 64 * waves_, 0, stream_>>>(v, bw, bw_recs_.p, cur, last);
       HIPCHK(hipGetLastError());
       end();
       prev = cur;
@@ -1823,22 +1837,31 @@ class HipBackend final : public Backend {
               r.nf = -1;
               continue;
             }
-            r.nf = e1 - e0;
-            for (int q = e0; q < e1; ++q) {
-              const gsk_csr_entry& e = s.csr[q];
-              const int k = q - e0;
-              int cls;
-              if (e.other < 0)
-                cls = s.bfaces[e.bface].kind == GSK_BC_PERIODIC ? 0 : 1;
-              else if (e.other < c0 || e.other >= c1)
-                cls = 3;
-              else
-                cls = (dirn == 0 ? pos[e.other] < pos[c] : pos[e.other] > pos[c]) ? 2 : 1;
-              r.code[k] = (e.other >= 0 ? e.other : 0) * 4 + cls;
-              r.sx[k] = static_cast<F>(e.s[0]);
-              r.sy[k] = static_cast<F>(e.s[1]);
-              r.sz[k] = static_cast<F>(e.s[2]);
+            r.row_w = static_cast<long long>(c) * chunk_ * static_cast<long long>(sizeof(F));
+            r.row_f = static_cast<long long>(c) * nv_ * static_cast<long long>(sizeof(F));
+            // slots: class 2 faces, then class 3, then class 1, each in CSR order; class 0 left out
+            int k = 0;
+            for (const int want : {2, 3, 1}) {
+              for (int q = e0; q < e1; ++q) {
+                const gsk_csr_entry& e = s.csr[q];
+                int cls;
+                if (e.other < 0)
+                  cls = s.bfaces[e.bface].kind == GSK_BC_PERIODIC ? 0 : 1;
+                else if (e.other < c0 || e.other >= c1)
+                  cls = 3;
+                else
+                  cls = (dirn == 0 ? pos[e.other] < pos[c] : pos[e.other] > pos[c]) ? 2 : 1;
+                if (cls != want) continue;
+                if (cls == 2) ++r.n_in;
+                if (cls == 3) ++r.n_out;
+                r.nb_w[k] = e.other >= 0 ? static_cast<long long>(e.other) * chunk_ * static_cast<long long>(sizeof(F)) : 0;
+                r.sx[k] = static_cast<F>(e.s[0]);
+                r.sy[k] = static_cast<F>(e.s[1]);
+                r.sz[k] = static_cast<F>(e.s[2]);
+                ++k;
+              }
             }
+            r.nf = k;
           }
           if (dirn == 0) total_levels += nlev;
         }
