@@ -836,8 +836,9 @@ __device__ __forceinline__ const F* byte_row(const F* base, i64 off) {
 // One cell from its sweep record in LDS: the results of forward_cell. The record's integers are
 // made wave-uniform (scalar registers), its areas are LDS operands. A neighbour row is loaded for
 // the whole wave when any lane is upwind of that face (one scalar branch, no per-lane masks), and
-// the accumulation selects per lane: adding +0 for a face a lane skips leaves every partial sum
-// unchanged bit for bit (the sums start at +0 and never become -0). The upwind sums L and Lout run
+// every lane adds every face, with max(a, 0) / min(a, 0) (times the neighbour value, which is
+// finite) in place of the reference's branches: a face a lane skips adds +-0, which leaves every
+// partial sum unchanged bit for bit (the sums start at +0 and never become -0). The upwind sums L and Lout run
 // over their faces in CSR order as in the reference; the outflow sum ap runs over the record's
 // grouped order, which can move it by an ulp. Unused slots have zero areas: a = +-0 adds +0 to ap
 // and nothing else. toff = the lane velocity's byte offset in a chunk row. HasPrev: a pass after
@@ -849,30 +850,33 @@ __device__ __forceinline__ void forward_rec(const SweepRec<F>& r, F x, F y, F z,
   const int n_in = uniform(r.n_in), n_io = n_in + uniform(r.n_out);
   const i64 row = uniform64(r.row_w);
   const F Ri = row_load(byte_row(rres, row), toff);
-  F ak[kGather], nb[kGather];
+  // max(a, 0) and min(a, 0) stand for "a > 0 ? a : 0" and "a > 0 ? 0 : a": they differ only in
+  // giving -0 for a = -0, and +-0 added to a sum that is never -0 leaves it unchanged. A face whose
+  // row is not loaded keeps nb = 0, so adding its am * nb = +-0 to an upwind sum changes nothing.
+  F am[kGather], nb[kGather];
+  F ap = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    const F a = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    ap += nmax(a, F(0));
+    am[k] = nmin(a, F(0));
     nb[k] = F(0);
     // one load site per face (in-block rows from this pass, out-of-block rows from the previous one)
     const bool load = HasPrev ? k < n_io : k < n_in;
     const F* base = (!HasPrev || k < n_in) ? static_cast<const F*>(cur) : prev;
-    if (load && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
-      nb[k] = row_load(byte_row(base, uniform64(r.nb_w[k])), toff);
+    if (load && __builtin_amdgcn_ballot_w64(a < F(0)) != 0) nb[k] = row_load(byte_row(base, uniform64(r.nb_w[k])), toff);
   }
-  F ap = F(0), am_out = F(0), L = F(0), Lout = F(0);
+  F am_out = F(0), L = F(0), Lout = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    const F a = ak[k];
-    const bool down = a > F(0);
-    ap += down ? a : F(0);
-    if (k < n_in) {
-      L += down ? F(0) : a * nb[k];
-    } else if (k < n_io) {
-      if (HasPrev)
-        Lout += down ? F(0) : a * nb[k];
+    if (HasPrev) {
+      if (k < n_in)
+        L += am[k] * nb[k];
       else
-        am_out -= down ? F(0) : a;
+        Lout += am[k] * nb[k];
+    } else {
+      L += am[k] * nb[k];
+      if (k >= n_in && k < n_io) am_out -= am[k];
     }
   }
   const F D = r.diag + diag_scale * ap + damping * am_out;
@@ -887,24 +891,21 @@ __device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z
   const int n_in = uniform(r.n_in), n_io = n_in + uniform(r.n_out);
   const i64 row = uniform64(r.row_w);
   const F own = row_load(byte_row(cur, row), toff);
-  F ak[kGather], nb[kGather];
+  F am[kGather], nb[kGather];  // as in forward_rec
+  F ap = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    const F a = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    ap += nmax(a, F(0));
+    am[k] = nmin(a, F(0));
     nb[k] = F(0);
-    if (k < n_in && __builtin_amdgcn_ballot_w64(!(ak[k] > F(0))) != 0)
-      nb[k] = row_load(byte_row(cur, uniform64(r.nb_w[k])), toff);
+    if (k < n_in && __builtin_amdgcn_ballot_w64(a < F(0)) != 0) nb[k] = row_load(byte_row(cur, uniform64(r.nb_w[k])), toff);
   }
-  F ap = F(0), am_out = F(0), U = F(0);
+  F am_out = F(0), U = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    const F a = ak[k];
-    const bool down = a > F(0);
-    ap += down ? a : F(0);
-    if (k < n_in)
-      U += down ? F(0) : a * nb[k];
-    else if (!HasPrev && k < n_io)
-      am_out -= down ? F(0) : a;
+    U += am[k] * nb[k];
+    if (!HasPrev && k >= n_in && k < n_io) am_out -= am[k];
   }
   const F D = r.diag + diag_scale * ap + damping * am_out;
   const F df = own - U / D;
