@@ -819,6 +819,12 @@ __device__ __forceinline__ void backward_cell(const View<F>& v, int cell, int t,
   if (update_f) v.f[static_cast<i64>(cell) * v.nv + p] += df;
 }
 
+// n / d as each precision's sweep computes it: FP64 divides (correctly rounded, the reference's
+// value); FP32 multiplies by the hardware reciprocal (v_rcp_f32, 1 ulp), two roundings instead of
+// the ~10-instruction correctly rounded division sequence, within the FP32 form's tolerance.
+__device__ __forceinline__ double sweep_div(double n, double d) { return n / d; }
+__device__ __forceinline__ float sweep_div(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
+
 __device__ __forceinline__ i64 uniform64(i64 x) {
   const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
   const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32)));
@@ -857,7 +863,7 @@ __device__ __forceinline__ void forward_rec(const SweepRec<F>& r, F x, F y, F z,
   F ap = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    const F a = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    const F a = dot3(x, y, z, r.sx[k], r.sy[k], r.sz[k]);
     ap += nmax(a, F(0));
     am[k] = nmin(a, F(0));
     nb[k] = F(0);
@@ -871,16 +877,16 @@ __device__ __forceinline__ void forward_rec(const SweepRec<F>& r, F x, F y, F z,
   for (int k = 0; k < kGather; ++k) {
     if (HasPrev) {
       if (k < n_in)
-        L += am[k] * nb[k];
+        L = madd(L, am[k], nb[k]);
       else
-        Lout += am[k] * nb[k];
+        Lout = madd(Lout, am[k], nb[k]);
     } else {
-      L += am[k] * nb[k];
+      L = madd(L, am[k], nb[k]);
       if (k >= n_in && k < n_io) am_out -= am[k];
     }
   }
-  const F D = r.diag + diag_scale * ap + damping * am_out;
-  row_ref(byte_row(cur, row), toff) = (-Ri - Lout - L) / D;
+  const F D = madd(madd(r.diag, diag_scale, ap), damping, am_out);
+  row_ref(byte_row(cur, row), toff) = sweep_div(-Ri - Lout - L, D);
 }
 
 // One cell from its sweep record in LDS: the results of backward_cell (same scheme as
@@ -895,7 +901,7 @@ __device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z
   F ap = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    const F a = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+    const F a = dot3(x, y, z, r.sx[k], r.sy[k], r.sz[k]);
     ap += nmax(a, F(0));
     am[k] = nmin(a, F(0));
     nb[k] = F(0);
@@ -904,11 +910,11 @@ __device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z
   F am_out = F(0), U = F(0);
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
-    U += am[k] * nb[k];
+    U = madd(U, am[k], nb[k]);
     if (!HasPrev && k >= n_in && k < n_io) am_out -= am[k];
   }
-  const F D = r.diag + diag_scale * ap + damping * am_out;
-  const F df = own - U / D;
+  const F D = madd(madd(r.diag, diag_scale, ap), damping, am_out);
+  const F df = own - sweep_div(U, D);
   row_ref(byte_row(cur, row), toff) = df;
   if (update_f) row_ref(byte_row(f, uniform64(r.row_f)), poff) += df;
 }

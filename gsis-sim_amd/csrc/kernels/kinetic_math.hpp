@@ -54,6 +54,15 @@ GSK_HD float nmax(float a, float b) { return ::fmaxf(a, b); }
 GSK_HD double nmin(double a, double b) { return ::fmin(a, b); }
 GSK_HD float nmin(float a, float b) { return ::fminf(a, b); }
 
+// Dot products and multiply-adds in the form each precision computes them: FP64 keeps the
+// reference's separately rounded products and left-to-right sums (the drop-in's parity at the ulp
+// level); FP32 fuses them (one v_fma_f32 per term), which the FP32 form's tolerance admits and which
+// cuts the vector instructions of these expressions by up to half.
+GSK_HD double dot3(double x, double y, double z, double a, double b, double c) { return x * a + y * b + z * c; }
+GSK_HD float dot3(float x, float y, float z, float a, float b, float c) { return ::fmaf(z, c, ::fmaf(y, b, x * a)); }
+GSK_HD double madd(double acc, double a, double b) { return acc + a * b; }
+GSK_HD float madd(float acc, float a, float b) { return ::fmaf(a, b, acc); }
+
 // f_eq = A exp(-B |xi - u|^2)
 template <class F>
 struct Maxw {
@@ -63,7 +72,7 @@ struct Maxw {
 template <class F>
 GSK_HD F maxw_eval(const Maxw<F>& m, F x, F y, F z) {
   const F cx = x - m.ux, cy = y - m.uy, cz = z - m.uz;
-  return m.A * dexp(-m.B * (cx * cx + cy * cy + cz * cz));
+  return m.A * dexp(-m.B * dot3(cx, cy, cz, cx, cy, cz));
 }
 
 // (rho, u, T) -> (A, B) with R = 1, computed in double and stored in F
@@ -99,15 +108,15 @@ struct CellP {
 
 template <class F>
 GSK_HD F shakhov_poly(const Target<F>& s, F cx, F cy, F cz, F c2) {
-  return s.coef * (cx * s.qx + cy * s.qy + cz * s.qz) * (c2 / s.theta - F(5));
+  return s.coef * dot3(cx, cy, cz, s.qx, s.qy, s.qz) * (c2 / s.theta - F(5));
 }
 
 template <class F>
 GSK_HD F target_eval(const Target<F>& s, F x, F y, F z) {
   const F cx = x - s.ucx, cy = y - s.ucy, cz = z - s.ucz;
-  const F c2 = cx * cx + cy * cy + cz * cz;
+  const F c2 = dot3(cx, cy, cz, cx, cy, cz);
   const F m = maxw_eval(s.eq, x, y, z);
-  return m * (F(1) + shakhov_poly(s, cx, cy, cz, c2) + (s.g0 + (cx * s.gx + cy * s.gy + cz * s.gz) + F(0.5) * s.g4 * c2));
+  return m * (F(1) + shakhov_poly(s, cx, cy, cz, c2) + madd(s.g0 + dot3(cx, cy, cz, s.gx, s.gy, s.gz), F(0.5) * s.g4, c2));
 }
 
 // cell params from 15 moments (kinetic_kernels.hpp:451-471)

@@ -249,9 +249,9 @@ __device__ __forceinline__ void limited_gradient_row(const View<F>& v, const Ell
     if ((skip >> k) & 1u) continue;
     if ((bnd >> k) & 1u) fk[k] = boundary_value_first_order(v, r.bface[k], r.sx[k], r.sy[k], r.sz[k], cell, p, x, y, z, fi);
     const F d = fk[k] - fi;
-    gx += r.cx[k] * d;
-    gy += r.cy[k] * d;
-    gz += r.cz[k] * d;
+    gx = madd(gx, r.cx[k], d);
+    gy = madd(gy, r.cy[k], d);
+    gz = madd(gz, r.cz[k], d);
     fmx = nmax(fmx, fk[k]);
     fmn = nmin(fmn, fk[k]);
   }
@@ -269,7 +269,7 @@ __device__ __forceinline__ void limited_gradient_row(const View<F>& v, const Ell
 #pragma unroll
   for (int k = 0; k < kGather; ++k) {
     if ((skip >> k) & 1u) continue;
-    const F proj = gx * r.dsx[k] + gy * r.dsy[k] + gz * r.dsz[k];
+    const F proj = dot3(gx, gy, gz, r.dsx[k], r.dsy[k], r.dsz[k]);
     proj_max = nmax(proj_max, proj);
     proj_min = nmin(proj_min, proj);
   }
@@ -479,7 +479,7 @@ __device__ __forceinline__ F residual_value_row(const View<F>& v, const Chunk<F>
     F fu[kGather], gux[kGather], guy[kGather], guz[kGather];
 #pragma unroll
     for (int k = 0; k < kGather; ++k) {
-      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      ak[k] = dot3(x, y, z, r.sx[k], r.sy[k], r.sz[k]);
       fu[k] = gux[k] = guy[k] = guz[k] = F(0);
       if (((skip | bnd) >> k) & 1u) continue;
       if (ak[k] < F(0)) {
@@ -491,18 +491,18 @@ __device__ __forceinline__ F residual_value_row(const View<F>& v, const Chunk<F>
       }
     }
 #pragma unroll
-    for (int k = 0; k < kGather; ++k) fup[k] = fu[k] + gux[k] * r.dox[k] + guy[k] * r.doy[k] + guz[k] * r.doz[k];
+    for (int k = 0; k < kGather; ++k) fup[k] = madd(madd(madd(fu[k], gux[k], r.dox[k]), guy[k], r.doy[k]), guz[k], r.doz[k]);
   } else {
     // FP64: the face's value is formed where its loads land (58 VGPRs instead of 104, so twice the
     // waves per SIMD hide the latency; all-loads-first measured 40% slower here)
 #pragma unroll
     for (int k = 0; k < kGather; ++k) {
-      ak[k] = x * r.sx[k] + y * r.sy[k] + z * r.sz[k];
+      ak[k] = dot3(x, y, z, r.sx[k], r.sy[k], r.sz[k]);
       fup[k] = F(0);
       if (((skip | bnd) >> k) & 1u) continue;
       if (ak[k] < F(0)) {
         const i64 wj = r.nbw[k] + t;
-        fup[k] = fp[r.nbf[k]] + ch.gx[wj] * r.dox[k] + ch.gy[wj] * r.doy[k] + ch.gz[wj] * r.doz[k];
+        fup[k] = madd(madd(madd(fp[r.nbf[k]], ch.gx[wj], r.dox[k]), ch.gy[wj], r.doy[k]), ch.gz[wj], r.doz[k]);
       }
     }
   }
@@ -513,7 +513,7 @@ __device__ __forceinline__ F residual_value_row(const View<F>& v, const Chunk<F>
     const F a = ak[k];
     F ff;
     if (a >= F(0)) {
-      ff = fi + gix * r.dsx[k] + giy * r.dsy[k] + giz * r.dsz[k];
+      ff = madd(madd(madd(fi, gix, r.dsx[k]), giy, r.dsy[k]), giz, r.dsz[k]);
     } else if (!((bnd >> k) & 1u)) {
       ff = fup[k];
     } else {
@@ -529,7 +529,7 @@ __device__ __forceinline__ F residual_value_row(const View<F>& v, const Chunk<F>
         ff = inflow_value(v.inflow[b], x, y, z);
       }
     }
-    R += a * ff;
+    R = madd(R, a, ff);
   }
   const CellP<F>& cp = v.cells[cell];
   R -= cp.coll * (target_eval(cp.s, x, y, z) - fi);
