@@ -39,13 +39,20 @@ def _package():
 
 
 WORKLOADS = {
-    # name: (tube args, velocity points per axis, [kinetic] settings); "mixed" = FP32 f with the
-    # matrix-core moment contraction
+    # name: (tube args, velocity points per axis, [kinetic] settings[, case settings]); "mixed" = FP32 f
+    # with the matrix-core moment contraction; case settings: delta (delta_rp, default 10)
     "tube40k_24cubed_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp64"}),
     "tube40k_24cubed_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32"}),
     "tube40k_24cubed_mixed": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32", "moment_sums": "mfma"}),
     "tube135k_32cubed_fp32": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32"}),
     "tube135k_32cubed_mixed": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32", "moment_sums": "mfma"}),
+    "tube135k_32cubed_fp32_delta1": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32"}, {"delta": 1.0}),
+    "tube135k_32cubed_fp32_delta100": (dict(n_core=24, n_ring=14, n_axial=70), 32, {"precision": "fp32"},
+                                       {"delta": 100.0}),
+    # 135k cells x 48^3 velocities (1.49e10 pairs, f alone 59 GB in FP32): the velocity chunk is
+    # narrowed automatically so the six chunk buffers fit beside f
+    "tube135k_48cubed_fp32_delta100": (dict(n_core=24, n_ring=14, n_axial=70), 48, {"precision": "fp32"},
+                                       {"delta": 100.0}),
 }
 
 # algorithmic bytes per (cell, velocity) pair and launch, in units of sizeof(f)
@@ -308,12 +315,14 @@ def main():
     if args.sensitivity:
         sensitivity_bench(g, mg, args)
         return
-    tube_args, nvel, kinetic = WORKLOADS[args.workload]
+    tube_args, nvel, kinetic = WORKLOADS[args.workload][:3]
+    case = WORKLOADS[args.workload][3] if len(WORKLOADS[args.workload]) > 3 else {}
+    delta = float(case.get("delta", 10.0))
     precision = kinetic["precision"]
     s_bytes = 8 if precision == "fp64" else 4
     with tempfile.TemporaryDirectory() as d:
         log(f"generating {args.workload}")
-        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=10.0, kinetic=dict(kinetic), **tube_args)
+        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=delta, kinetic=dict(kinetic), **tube_args)
         setup = g.Setup(path)
         log(f"setup: {setup.n_cells} cells, {setup.nv} velocities, {setup.n_blocks} blocks")
         sv = g.Solver(setup)
@@ -339,6 +348,8 @@ def main():
             log(f"step {i + 1}/{args.steps}: residual {info['residual']:.4e}")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        free_b, _ = torch.cuda.mem_get_info()
+        memory["device_used_after_bytes"] = int(total_b - free_b)
         chunk = be.chunk_width()
         n_chunks = -(-setup.nv // chunk)
         sv.close()
@@ -363,7 +374,7 @@ def main():
             "data": "synthetic",
             "config": {"workload": args.workload, "cells": setup.n_cells, "velocities": setup.nv,
                        "inner_sweeps": inner, "block_passes": passes, "sweep_blocks": setup.n_blocks,
-                       "chunk_width": chunk, "delta_rp": 10.0},
+                       "chunk_width": chunk, "delta_rp": delta},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": measured_traffic(args.workload, dom, precision),
