@@ -73,6 +73,7 @@ class SetupView(C.Structure):
         ("cold_bgk_temperature", C.c_double), ("mu0", C.c_double), ("omega", C.c_double), ("Pr", C.c_double),
         ("block_passes", C.c_int32), ("negative_check", C.c_int32), ("precision_fp64", C.c_int32),
         ("moments_mfma", C.c_int32),
+        ("weight_rel", C.POINTER(C.c_double)), ("grid_nodes", C.POINTER(C.c_double)),
     ]
 
 

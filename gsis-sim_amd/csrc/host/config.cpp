@@ -392,6 +392,12 @@ CaseConfig parse_case_toml(const std::string& text, const std::string& source) {
       total *= cfg.vn[static_cast<std::size_t>(a)];
     }
     GSK_CHECK(total <= 100000000, "too many velocity points");
+    // extension: tanh-stretched grid (no reference counterpart; 0 keeps the uniform midpoint grid)
+    if (const TomlValue* st = r.find("stretch")) {
+      cfg.vstretch = st->number();
+      GSK_CHECK(std::isfinite(cfg.vstretch) && cfg.vstretch >= 0.0 && cfg.vstretch <= 10.0,
+                "velocity_grid.stretch must be in [0, 10]");
+    }
     r.finish();
   }
   if (doc.tables.count("gsis")) {

@@ -63,6 +63,7 @@ struct CaseConfig {
   // [velocity_grid]
   std::array<int, 3> vn{};
   std::array<double, 3> vlo{}, vhi{};
+  double vstretch = 0.0;  // velocity_grid.stretch: tanh stretching strength (0 = the reference's uniform grid)
   // [kinetic] raw values (validated by parse_kinetic_params)
   TomlTable kinetic;
   // [gsis] values used by CIS-only runs

@@ -7,8 +7,9 @@
 
 namespace gsk {
 
-VelocityGrid::VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::array<double, 3> hi)
-    : n_(n), lo_(lo), hi_(hi) {
+VelocityGrid::VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::array<double, 3> hi, double stretch)
+    : n_(n), lo_(lo), hi_(hi), stretch_(stretch) {
+  GSK_CHECK(std::isfinite(stretch_) && stretch_ >= 0.0, "invalid velocity grid stretch");
   long long total = 1;
   for (std::size_t a = 0; a < 3; ++a) {
     GSK_CHECK(n_[a] >= 1, "velocity grid needs at least one point per axis");
@@ -21,6 +22,24 @@ VelocityGrid::VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::
   GSK_CHECK(total <= 100000000, "too many velocity points");
   size_ = static_cast<int>(total);
   weight_ = delta_[0] * delta_[1] * delta_[2];
+  for (std::size_t a = 0; a < 3; ++a) {
+    const int na = n_[a];
+    node_[a].resize(static_cast<std::size_t>(na));
+    axis_w_[a].resize(static_cast<std::size_t>(na));
+    const double h = 0.5 * (hi_[a] - lo_[a]), tb = std::tanh(stretch_);
+    for (int i = 0; i < na; ++i) {
+      const int m = 2 * i + 1 - na;  // s_i = m / na; m and the mirror's -m are exact negatives
+      if (!stretched()) {
+        node_[a][static_cast<std::size_t>(i)] = center_[a] + static_cast<double>(m) * half_delta_[a];
+        axis_w_[a][static_cast<std::size_t>(i)] = delta_[a];
+        continue;
+      }
+      const double st = (static_cast<double>(std::abs(m)) / na) * tb;  // |s_i| tanh b
+      const double off = (h / stretch_) * std::atanh(st);
+      node_[a][static_cast<std::size_t>(i)] = center_[a] + (m < 0 ? -off : off);
+      axis_w_[a][static_cast<std::size_t>(i)] = (2.0 / na) * (h / stretch_) * tb / (1.0 - st * st);
+    }
+  }
   xs_.resize(static_cast<std::size_t>(size_));
   ys_.resize(static_cast<std::size_t>(size_));
   zs_.resize(static_cast<std::size_t>(size_));
@@ -33,6 +52,14 @@ VelocityGrid::VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::
         zs_[k] = coordinate(2, l);
       }
   for (std::size_t a = 0; a < 3; ++a) symmetric_[a] = std::abs(lo_[a] + hi_[a]) <= 1e-12 * (hi_[a] - lo_[a]);
+}
+
+double VelocityGrid::max_spacing(int a) const {
+  const auto s = static_cast<std::size_t>(a);
+  double d = delta_[s];
+  if (stretched())
+    for (std::size_t i = 0; i < axis_w_[s].size(); ++i) d = std::max(d, axis_w_[s][i]);
+  return d;
 }
 
 std::vector<Vec3> make_sweep_directions(const std::string& order) {
@@ -118,6 +145,13 @@ VelocityLayout make_velocity_layout(const VelocityGrid& grid, const std::vector<
     L.xi_z[static_cast<std::size_t>(p)] = xi.z;
     L.xi_max = std::max(L.xi_max, norm(xi));
   }
+  if (grid.stretched()) {
+    L.weight_rel.resize(static_cast<std::size_t>(L.nv));
+    for (int p = 0; p < L.nv; ++p)
+      L.weight_rel[static_cast<std::size_t>(p)] = grid.relative_weight(L.storage_to_grid[static_cast<std::size_t>(p)]);
+    for (int a = 0; a < 3; ++a)
+      for (int i = 0; i < grid.n(a); ++i) L.nodes.push_back(grid.coordinate(a, i));
+  }
   for (const int a : axes)
     for (int p = 0; p < L.nv; ++p) {
       const int q = p ^ L.axis_xor[static_cast<std::size_t>(a)];
@@ -188,7 +222,7 @@ KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh,
   s.derived = derive_quantities(config);
   s.length_scale = config.L_ref;
   const DerivedQuantities& ref = s.derived;
-  const VelocityGrid grid(config.vn, config.vlo, config.vhi);
+  const VelocityGrid grid(config.vn, config.vlo, config.vhi, config.vstretch);
   const MeshGeometry geometry0 = compute_geometry(mesh);
   {
     const std::vector<std::string> errors = check_case_against_mesh(config, mesh, geometry0, grid);
@@ -311,7 +345,7 @@ KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh,
   for (int a = 0; a < 3; ++a) {
     s.deq_lo[a] = grid.lo(a);
     s.deq_hi[a] = grid.hi(a);
-    s.deq_d[a] = grid.delta(a);
+    s.deq_d[a] = grid.max_spacing(a);
   }
   s.deq_tolerance = params.precision == "fp64" ? 1e-11 : 1e-6;
   s.sweep_dirs = make_sweep_directions(params.sweep_order);
@@ -471,6 +505,8 @@ gsk_setup_view KineticSetup::view() const {
   const std::string& rule = params.gsis_negative_check;
   v.negative_check = rule == "created" ? 1 : rule == "limit" ? 2 : rule == "blend" ? 3 : 0;
   v.precision_fp64 = params.precision == "fp64" ? 1 : 0;
+  v.weight_rel = velocity.weight_rel.empty() ? nullptr : velocity.weight_rel.data();
+  v.grid_nodes = velocity.nodes.empty() ? nullptr : velocity.nodes.data();
   v.moments_mfma = params.moment_sums == "mfma" ? 1 : 0;
   return v;
 }

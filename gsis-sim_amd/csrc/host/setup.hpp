@@ -30,18 +30,33 @@ enum BoundaryKind : std::int32_t {
 
 class VelocityGrid {
  public:
-  VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::array<double, 3> hi);
+  // stretch = 0: the reference's uniform midpoint grid. stretch = b > 0 (an extension): per axis,
+  // points uniform in s = tanh(b (xi - c) / h) / tanh(b) over the range c +- h, i.e.
+  // xi_i = c + (h / b) atanh(s_i tanh b) at the midpoints s_i of n equal cells of (-1, 1), each with
+  // the weight (dxi/ds) ds = (2 / n) (h / b) tanh b / (1 - (s_i tanh b)^2): the midpoint rule in s,
+  // clustering the points near the centre and widening them towards the range ends.
+  VelocityGrid(std::array<int, 3> n, std::array<double, 3> lo, std::array<double, 3> hi, double stretch = 0.0);
   int size() const { return size_; }
   int n(int a) const { return n_[static_cast<std::size_t>(a)]; }
   double lo(int a) const { return lo_[static_cast<std::size_t>(a)]; }
   double hi(int a) const { return hi_[static_cast<std::size_t>(a)]; }
   double delta(int a) const { return delta_[static_cast<std::size_t>(a)]; }
   double weight() const { return weight_; }
-  // lo + (i + 1/2) d, written about the range centre so mirrored points are exactly -xi
-  double coordinate(int a, int i) const {
-    const auto s = static_cast<std::size_t>(a);
-    return center_[s] + static_cast<double>(2 * i + 1 - n_[s]) * half_delta_[s];
+  double stretch() const { return stretch_; }
+  bool stretched() const { return stretch_ > 0.0; }
+  // node i of axis a: lo + (i + 1/2) d on the uniform grid, written about the range centre so
+  // mirrored points are exactly -xi (the stretched nodes are odd about the centre as well)
+  double coordinate(int a, int i) const { return node_[static_cast<std::size_t>(a)][static_cast<std::size_t>(i)]; }
+  // quadrature weight of point k relative to weight() (exactly 1 on the uniform grid)
+  double relative_weight(int k) const {
+    if (!stretched()) return 1.0;
+    const std::array<int, 3> ijk = multi_index(k);
+    double w = 1.0;
+    for (std::size_t a = 0; a < 3; ++a) w *= axis_w_[a][static_cast<std::size_t>(ijk[a])];
+    return w / weight_;
   }
+  // largest node spacing of axis a (the spacing the discrete-equilibrium resolution test uses)
+  double max_spacing(int a) const;
   int index(int i, int j, int l) const { return i + n_[0] * (j + n_[1] * l); }
   std::array<int, 3> multi_index(int k) const { return {k % n_[0], (k / n_[0]) % n_[1], k / (n_[0] * n_[1])}; }
   Vec3 xi(int k) const { return {xs_[static_cast<std::size_t>(k)], ys_[static_cast<std::size_t>(k)], zs_[static_cast<std::size_t>(k)]}; }
@@ -55,7 +70,8 @@ class VelocityGrid {
  private:
   std::array<int, 3> n_{};
   std::array<double, 3> lo_{}, hi_{}, delta_{}, center_{}, half_delta_{};
-  double weight_ = 0.0;
+  double weight_ = 0.0, stretch_ = 0.0;
+  std::array<std::vector<double>, 3> node_, axis_w_;
   int size_ = 0;
   std::vector<double> xs_, ys_, zs_;
   std::array<bool, 3> symmetric_{};
@@ -68,6 +84,8 @@ struct VelocityLayout {
   std::vector<int> storage_to_grid, grid_to_storage;
   std::vector<double> xi_x, xi_y, xi_z;
   double weight = 0.0;
+  std::vector<double> weight_rel;  // storage order, w_p / weight (empty on the uniform grid)
+  std::vector<double> nodes;       // stretched grid: node coordinates of axis 0, 1, 2 (empty: uniform)
   double xi_max = 0.0;
 };
 

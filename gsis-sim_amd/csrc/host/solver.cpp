@@ -42,7 +42,8 @@ int discrete_maxwell_host(const KineticSetup& s, double tol, double rho, double 
     for (int p = 0; p < nv; ++p) {
       const double x = s.velocity.xi_x[static_cast<std::size_t>(p)], y = s.velocity.xi_y[static_cast<std::size_t>(p)],
                    z = s.velocity.xi_z[static_cast<std::size_t>(p)];
-      const double e = dev::maxw_eval(m, x, y, z);
+      double e = dev::maxw_eval(m, x, y, z);
+      if (!s.velocity.weight_rel.empty()) e *= s.velocity.weight_rel[static_cast<std::size_t>(p)];  // stretched grid
       dev::deq_terms<double>(e, 0.0, x - ux, y - uy, z - uz, t);
       for (int k = 0; k < dev::kDeqSums; ++k) sums[k] += t[k];
     }
@@ -270,7 +271,8 @@ std::array<double, 10> KineticSolver::compute_wall_unit_sums(Index b) const {  /
                   s_.velocity.xi_z[static_cast<std::size_t>(p)]};
     const double un = dot(xi, n);
     if (un >= 0.0) continue;
-    const double wf = s_.velocity.weight * dev::maxw_eval(m, xi.x, xi.y, xi.z);
+    double wf = s_.velocity.weight * dev::maxw_eval(m, xi.x, xi.y, xi.z);
+    if (!s_.velocity.weight_rel.empty()) wf *= s_.velocity.weight_rel[static_cast<std::size_t>(p)];  // stretched grid
     const double psi[5] = {1.0, xi.x, xi.y, xi.z, 0.5 * norm2(xi)};
     for (int k = 0; k < 5; ++k) {
       out[static_cast<std::size_t>(k)] += wf * un * psi[k];

@@ -82,7 +82,8 @@ __device__ __forceinline__ F block_extreme(F v, F* lds) {
 constexpr int kDeqNewtonSums = 15;
 template <class F>
 __device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F* __restrict__ ys,
-                                      const F* __restrict__ zs, double weight, const DeqGrid& grid, double tolerance,
+                                      const F* __restrict__ zs, double weight, const F* __restrict__ wq,
+                                      const DeqGrid& grid, double tolerance,
                                       double rho, double ux, double uy, double uz, double T, const Target<F>* poly,
                                       Maxw<F>& out, double (&gamma)[5], double* lds) {
   for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
@@ -101,7 +102,9 @@ __device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F*
       const F x = xs[p], y = ys[p], z = zs[p];
       const F cx = x - fux, cy = y - fuy, cz = z - fuz;
       F t[kDeqSums];
-      deq_terms<F>(maxw_eval(m, x, y, z), F(0), cx, cy, cz, t);
+      F e = maxw_eval(m, x, y, z);
+      if (wq != nullptr) e *= wq[p];  // stretched grid: weight * wq[p] per velocity
+      deq_terms<F>(e, F(0), cx, cy, cz, t);
 #pragma unroll
       for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] += static_cast<double>(t[k]);
     }
@@ -142,7 +145,9 @@ __device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F*
           const F cx = x - fux, cy = y - fuy, cz = z - fuz;
           const F pm = shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz);
           F t[kDeqSums];
-          deq_terms<F>(maxw_eval(m, x, y, z), pm, cx, cy, cz, t);
+          F e = maxw_eval(m, x, y, z);
+          if (wq != nullptr) e *= wq[p];
+          deq_terms<F>(e, pm, cx, cy, cz, t);
 #pragma unroll
           for (int k = 0; k < 5; ++k) s5[k] += static_cast<double>(t[kDeqNewtonSums + k]);
         }
@@ -179,8 +184,8 @@ __device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, d
   if (!v.discrete_eq) return maxw_from_state<F>(rho, ux, uy, uz, T);
   Maxw<F> m;
   double gamma[5];
-  const int r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.deq, v.deq.tolerance, rho, ux, uy,
-                                          uz, T, nullptr, m, gamma, lds);
+  const int r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, rho,
+                                          ux, uy, uz, T, nullptr, m, gamma, lds);
   if (result) *result = r;
   return m;
 }
@@ -193,8 +198,9 @@ __device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, d
 // maxwell_params_to_state, kinetic_kernels.hpp:543-552). Newton tolerance tol (1e-10 as the reference).
 __global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double* __restrict__ xd,
                                                            const double* __restrict__ yd, const double* __restrict__ zd,
-                                                           double weight, DeqGrid grid, double tol,
-                                                           double* __restrict__ states, int n) {
+                                                           double weight, const double* __restrict__ wqd,
+                                                           DeqGrid grid, double tol, double* __restrict__ states,
+                                                           int n) {
   __shared__ double lds[kWaves * kDeqSums];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     double* st = states + static_cast<i64>(i) * 6;
@@ -202,8 +208,8 @@ __global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double*
     if (!(rho > 0.0) || !(T > 0.0)) continue;
     Maxw<double> m;
     double gamma[5];
-    const int r = block_discrete_maxwell<double>(nv, xd, yd, zd, weight, grid, tol, rho, ux, uy, uz, T, nullptr, m,
-                                                 gamma, lds);
+    const int r = block_discrete_maxwell<double>(nv, xd, yd, zd, weight, wqd, grid, tol, rho, ux, uy, uz, T, nullptr,
+                                                 m, gamma, lds);
     if (r > 0 && threadIdx.x == 0) {
       const double Tn = 1.0 / (2.0 * m.B);
       const double tpt = 2.0 * kPi * Tn;
@@ -246,7 +252,9 @@ __global__ __launch_bounds__(kBlock) void k_moments(View<F> v, const double* __r
     const F* fc = v.f + static_cast<i64>(c) * v.nv;
     for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
       double t[kMomentSums];
-      moment_terms<double>(static_cast<double>(fc[p]), xd[p] - u0x, yd[p] - u0y, zd[p] - u0z, t);
+      double fp = static_cast<double>(fc[p]);
+      if (v.wqd != nullptr) fp *= v.wqd[p];  // stretched grid: weight * wqd[p] per velocity
+      moment_terms<double>(fp, xd[p] - u0x, yd[p] - u0y, zd[p] - u0z, t);
       for (int k = 0; k < kMomentSums; ++k) s[k] += t[k];
     }
     block_sum<kMomentSums>(s, lds);
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(256) void k_moments_mfma(View<float> v, double* __r
   floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int k0 = k_begin; k0 < k_end; k0 += 16) {
     const int pb = k0 + 4 * g;  // this lane's 4 velocities pb .. pb + 3 (one per MFMA)
-    float fv[4], xv[4], yv[4], zv[4];
+    float fv[4], xv[4], yv[4], zv[4], wv[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     if (vec4 && pb + 3 < k_end) {
       const float4 f4 = *reinterpret_cast<const float4*>(frow + pb);
       const float4 x4 = *reinterpret_cast<const float4*>(v.xi_x + pb);
@@ -316,6 +324,10 @@ __global__ __launch_bounds__(256) void k_moments_mfma(View<float> v, double* __r
       xv[0] = x4.x, xv[1] = x4.y, xv[2] = x4.z, xv[3] = x4.w;
       yv[0] = y4.x, yv[1] = y4.y, yv[2] = y4.z, yv[3] = y4.w;
       zv[0] = z4.x, zv[1] = z4.y, zv[2] = z4.z, zv[3] = z4.w;
+      if (v.wq != nullptr) {
+        const float4 w4 = *reinterpret_cast<const float4*>(v.wq + pb);
+        wv[0] = w4.x, wv[1] = w4.y, wv[2] = w4.z, wv[3] = w4.w;
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -326,6 +338,7 @@ __global__ __launch_bounds__(256) void k_moments_mfma(View<float> v, double* __r
         xv[j] = v.xi_x[q];
         yv[j] = v.xi_y[q];
         zv[j] = v.xi_z[q];
+        if (v.wq != nullptr) wv[j] = v.wq[q];
       }
     }
 #pragma unroll
@@ -334,7 +347,7 @@ __global__ __launch_bounds__(256) void k_moments_mfma(View<float> v, double* __r
       const float e1 = ax1 == 1 ? cx : ax1 == 2 ? cy : ax1 == 3 ? cz : base;
       const float e2 = ax2 == 1 ? cx : ax2 == 2 ? cy : ax2 == 3 ? cz : 1.0f;
       const float e3 = cube ? cx * cx + cy * cy + cz * cz : 1.0f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(e1 * e2 * e3, fv[j], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(e1 * e2 * e3 * wv[j], fv[j], acc, 0, 0, 0);
     }
     if ((k0 + 16 - k_begin) % kMfmaBlock == 0) {
 #pragma unroll
@@ -392,8 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double*
     if (v.discrete_eq) {
       Maxw<F> eq;
       double gamma[5];
-      r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.deq, v.deq.tolerance, m[0], m[1], m[2],
-                                    m[3], m[4], &cp.s, eq, gamma, lds);
+      r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
+                                    m[2], m[3], m[4], &cp.s, eq, gamma, lds);
       if (r > 0) {
         cp.s.eq = eq;
         cp.s.g0 = static_cast<F>(gamma[0]);
@@ -459,6 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary_sums(View<F> v, const int* 
       } else {
         fb = maxw_eval(v.inflow[b].m, x, y, z);
       }
+      if (v.wq != nullptr) fb *= v.wq[p];  // stretched grid
       const F wf = v.weight * fb;
       const F e = F(0.5) * (x * x + y * y + z * z);
       const F af = an * wf;
@@ -503,8 +517,11 @@ __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int*
     const int c = cells[i];
     const CsrE<F> e = v.csr[entries[i]];
     double acc[1] = {0.0};
-    for (int p = threadIdx.x; p < v.nv; p += blockDim.x)
-      acc[0] += static_cast<double>(internal_face_flux_term(v, c, e, p, v.xi_x[p], v.xi_y[p], v.xi_z[p]));
+    for (int p = threadIdx.x; p < v.nv; p += blockDim.x) {
+      double term = static_cast<double>(internal_face_flux_term(v, c, e, p, v.xi_x[p], v.xi_y[p], v.xi_z[p]));
+      if (v.wqd != nullptr) term *= v.wqd[p];  // stretched grid
+      acc[0] += term;
+    }
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) out[i] = static_cast<double>(v.weight) * acc[0];
   }
@@ -584,6 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int
     const F R = (r.skip | r.bnd) == 0 ? residual_value_row<true>(v, ch, r, c, t) : residual_value_row<false>(v, ch, r, c, t);
     rres[static_cast<i64>(c) * ch.stride + t] = R;
     r2[0] = static_cast<double>(R) * static_cast<double>(R);
+    if (v.wqd != nullptr) r2[0] *= v.wqd[ch.k0 + t];  // stretched grid: weighted sum of R^2
   }
   block_sum<1>(r2, lds);
   if (threadIdx.x == 0) partial[static_cast<i64>(c) * tiles + tile] = r2[0];
@@ -602,6 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_csr(View<F> v, Chunk<F> ch,
       const F R = residual_value(v, ch, c, t);
       rres[static_cast<i64>(c) * ch.stride + t] = R;
       r2[0] = static_cast<double>(R) * static_cast<double>(R);
+      if (v.wqd != nullptr) r2[0] *= v.wqd[ch.k0 + t];
     }
     block_sum<1>(r2, lds);
     if (threadIdx.x == 0) partial[static_cast<i64>(c) * tiles + tile] = r2[0];
@@ -1202,6 +1221,14 @@ class HipBackend final : public Backend {
     xi_y_.upload(yf.data(), yf.size(), stream_);
     xi_z_.upload(zf.data(), zf.size(), stream_);
     xd_.upload(s.xi_x, static_cast<std::size_t>(nv_), stream_);
+    if (s.weight_rel != nullptr) {  // stretched velocity grid
+      std::vector<F> wq(static_cast<std::size_t>(nv_));
+      for (int p = 0; p < nv_; ++p) wq[static_cast<std::size_t>(p)] = static_cast<F>(s.weight_rel[p]);
+      wq_.upload(wq.data(), wq.size(), stream_);
+      wqd_.upload(s.weight_rel, static_cast<std::size_t>(nv_), stream_);
+      GSK_CHECK(s.grid_nodes != nullptr, "a stretched velocity grid needs its node coordinates");
+      nodes_.upload(s.grid_nodes, static_cast<std::size_t>(s.grid_n[0]) + s.grid_n[1] + s.grid_n[2], stream_);
+    }
     yd_.upload(s.xi_y, static_cast<std::size_t>(nv_), stream_);
     zd_.upload(s.xi_z, static_cast<std::size_t>(nv_), stream_);
     // mesh tables
@@ -1514,7 +1541,7 @@ class HipBackend final : public Backend {
     begin(kOther);
     inflow_stage_.upload(reinterpret_cast<const double*>(states), n * 6, stream_);
     dev::k_inflow_correct<<<static_cast<unsigned>(std::min<std::size_t>(n, 65535)), dev::kBlock, 0, stream_>>>(
-        nv_, xd_.p, yd_.p, zd_.p, weight_, view().deq, inflow_tol_, inflow_stage_.p, static_cast<int>(n));
+        nv_, xd_.p, yd_.p, zd_.p, weight_, wqd_.p, view().deq, inflow_tol_, inflow_stage_.p, static_cast<int>(n));
     HIPCHK(hipGetLastError());
     end();
     download(reinterpret_cast<double*>(states), inflow_stage_.p, n * 6);
@@ -2002,6 +2029,9 @@ class HipBackend final : public Backend {
     v.xi_z = xi_z_.p;
     v.weight = static_cast<F>(weight_);
     v.weight_d = weight_;
+    v.wq = wq_.p;
+    v.wqd = wqd_.p;
+    v.nodes = nodes_.p;
     v.csr_off = csr_off_.p;
     v.csr = csr_.p;
     v.ell = ell_.p;
@@ -2040,6 +2070,8 @@ class HipBackend final : public Backend {
   dev::DeqGrid deq_{};
   DBuf<F> xi_x_, xi_y_, xi_z_;
   DBuf<double> xd_, yd_, zd_;
+  DBuf<F> wq_;                   // stretched grid only (relative weights; empty: uniform)
+  DBuf<double> wqd_, nodes_;
   DBuf<int> csr_off_;
   DBuf<dev::CsrE<F>> csr_;
   DBuf<dev::EllRow<F>> ell_;

@@ -59,6 +59,12 @@ struct View {
   const F* xi_z;
   F weight;
   double weight_d;
+  // stretched velocity grid (an extension; all three null on the reference's uniform grid): the
+  // quadrature weight of velocity p is weight * wq[p] (wqd: the same in FP64), and nodes holds the
+  // node coordinates of axes x, y, z back to back for the Maxwell-wall interpolation
+  const F* wq;
+  const double* wqd;
+  const double* nodes;
   const int* csr_off;
   const CsrE<F>* csr;
   const EllRow<F>* ell;  // [cell] padded face rows (cells with more than kGather faces: CSR path)
@@ -299,7 +305,28 @@ __device__ F reflected_value(const View<F>& v, const BFace<F>& b, int p, F x, F 
   const F r[3] = {x - F(2) * un * b.nx, y - F(2) * un * b.ny, z - F(2) * un * b.nz};
   int base[3];
   F t[3];
+  int node_off = 0;
   for (int a = 0; a < 3; ++a) {
+    if (v.nodes != nullptr) {
+      // stretched grid: the bracketing nodes by bisection, linear in xi between them; outside the
+      // node range (by more than 1e-5 of the end spacing, the uniform form's slack) the value is 0
+      const double* nd = v.nodes + node_off;
+      const int n = v.grid_n[a];
+      node_off += n;
+      const double ra = static_cast<double>(r[a]);
+      if (ra < nd[0] - 1e-5 * (nd[1] - nd[0]) || ra > nd[n - 1] + 1e-5 * (nd[n - 1] - nd[n - 2])) return F(0);
+      int lo = 0, hi = n - 1;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (nd[mid] <= ra)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      base[a] = lo;
+      t[a] = static_cast<F>(dmax(0.0, dmin(1.0, (ra - nd[lo]) / (nd[lo + 1] - nd[lo]))));
+      continue;
+    }
     const double zz = (static_cast<double>(r[a]) - v.deq.lo[a]) / v.deq.d[a] - 0.5;
     if (zz < -1e-5 || zz > v.grid_n[a] - 1.0 + 1e-5) return F(0);
     const double cl = dmax(0.0, dmin(zz, v.grid_n[a] - 1.0));

@@ -276,8 +276,9 @@ HELIUM = dict(molecular_mass=6.65e-27, mu_ref=1.916e-5, T_ref=273.15, omega=0.66
 
 
 def write_case(path: str, mesh_file: str, name: str, boundaries: list, grid_n, grid_range, conditions: dict,
-               kinetic: dict | None = None, gas: dict | None = None) -> None:
-    """Write a case TOML in the reference's schema (src/config/case_config.cpp:312-460)."""
+               kinetic: dict | None = None, gas: dict | None = None, stretch: float = 0.0) -> None:
+    """Write a case TOML in the reference's schema (src/config/case_config.cpp:312-460); stretch > 0 adds
+    velocity_grid.stretch, this build's tanh-stretched velocity grid (no reference counterpart)."""
     g = dict(HELIUM, **(gas or {}))
     lines = ["[case]", f'name = "{name}"', f'mesh = "{mesh_file}"', 'mode = "cis"', "", "[gas]"]
     lines += [f"{k} = {v!r}" for k, v in g.items()]
@@ -293,6 +294,8 @@ def write_case(path: str, mesh_file: str, name: str, boundaries: list, grid_n, g
                 lines.append(f"{k} = {v!r}")
     lines += ["", "[velocity_grid]", f"n = [{', '.join(str(int(x)) for x in grid_n)}]",
               "range = [" + ", ".join(f"[{lo!r}, {hi!r}]" for lo, hi in grid_range) + "]"]
+    if stretch > 0.0:
+        lines.append(f"stretch = {float(stretch)!r}")
     if kinetic:
         lines += ["", "[kinetic]"]
         for k, v in kinetic.items():
@@ -326,7 +329,7 @@ def slab_case(directory: str, nx: int = 200, n_vel: int = 24, vrange: float = 6.
 
 def tube_case(directory: str, n_core: int = 16, n_ring: int = 10, n_axial: int = 45, n_vel: int = 24,
               vrange: float = 6.0, delta: float = 10.0, accommodation: float = 1.0, kinetic: dict | None = None,
-              name: str = "tube") -> str:
+              name: str = "tube", stretch: float = 0.0) -> str:
     """Tube (R = L_ref, length 5R) from a pressure inlet (p_in, T0) to vacuum, diffuse (or Maxwell) wall."""
     import os
     mesh_file = f"{name}_{n_core}_{n_ring}_{n_axial}.pmesh"
@@ -338,5 +341,6 @@ def tube_case(directory: str, n_core: int = 16, n_ring: int = 10, n_axial: int =
     path = os.path.join(directory, f"{name}.toml")
     write_case(path, mesh_file, name,
                [dict(patch="inlet", type="pressure_inlet", temperature=T0), dict(patch="outlet", type="vacuum"), wall],
-               [n_vel] * 3, [(-vrange, vrange)] * 3, dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), kinetic)
+               [n_vel] * 3, [(-vrange, vrange)] * 3, dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), kinetic,
+               stretch=stretch)
     return path

@@ -123,6 +123,12 @@ typedef struct gsk_setup_view {
   int32_t precision_fp64;       /* 1: f stored and swept in FP64; 0: FP32 */
   int32_t moments_mfma;         /* [kinetic].moment_sums == "mfma" (FP32 form only): moment sums as an
                                    FP32 matrix-core contraction, added in FP64 (no reference counterpart) */
+  /* [velocity_grid].stretch > 0 (no reference counterpart): per-velocity quadrature weights as
+   * weight * weight_rel[p] (storage order), and the node coordinates of axes x, y, z
+   * (grid_n[0] + grid_n[1] + grid_n[2] values) for the Maxwell-wall interpolation. Both NULL on the
+   * reference's uniform grid. */
+  const double* weight_rel;
+  const double* grid_nodes;
 } gsk_setup_view;
 
 const char* gsk_last_error(void);

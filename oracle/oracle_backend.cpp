@@ -249,8 +249,9 @@ inline bool accept(int r, double rel, double rel_prev, bool last) {
 
 // discrete_maxwell_seq (kinetic_kernels.hpp:499-540)
 template <class F>
-int discrete_maxwell(const F* xx, const F* xy, const F* xz, int nv, double w, const Grid& g, double rho, double ux,
-                     double uy, double uz, double T, const Shakhov<F>* poly, MaxwellParams<F>& out, double* gamma) {
+int discrete_maxwell(const F* xx, const F* xy, const F* xz, const F* wq, int nv, double w, const Grid& g, double rho,
+                     double ux, double uy, double uz, double T, const Shakhov<F>* poly, MaxwellParams<F>& out,
+                     double* gamma) {
   for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
   out = params_from<F>(rho, ux, uy, uz, T);
   if (resolved(g, ux, uy, uz, T)) return 0;
@@ -264,7 +265,8 @@ int discrete_maxwell(const F* xx, const F* xy, const F* xz, int nv, double w, co
     F t[20];
     for (int p = 0; p < nv; ++p) {
       const F cx = xx[p] - fux, cy = xy[p] - fuy, cz = xz[p] - fuz;
-      const F e = maxwellian(m, xx[p], xy[p], xz[p]);
+      F e = maxwellian(m, xx[p], xy[p], xz[p]);
+      if (wq) e *= wq[p];  // stretched grid: quadrature weight w * wq[p]
       const F pm = poly ? shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz) : F(0);
       newton_terms<F>(e, pm, cx, cy, cz, t);
       for (int k = 0; k < 20; ++k) sums[k] += double(t[k]);
@@ -326,6 +328,12 @@ class Backend {
       xz_.push_back(F(zd_[p]));
     }
     w_ = v.weight;
+    if (v.weight_rel != nullptr) {  // stretched velocity grid (an extension of the reference's uniform grid)
+      wqd_.assign(v.weight_rel, v.weight_rel + nv_);
+      for (int p = 0; p < nv_; ++p) wq_.push_back(F(wqd_[p]));
+      check(v.grid_nodes != nullptr, "stretched grid without node coordinates");
+      nodes_.assign(v.grid_nodes, v.grid_nodes + v.grid_n[0] + v.grid_n[1] + v.grid_n[2]);
+    }
     off_.assign(v.csr_offsets, v.csr_offsets + nc_ + 1);
     for (int k = 0; k < off_[nc_]; ++k) {
       const gsk_csr_entry& e = v.csr[k];
@@ -440,7 +448,8 @@ class Backend {
       const double u0x = shift_[c * 3], u0y = shift_[c * 3 + 1], u0z = shift_[c * 3 + 2];
       double s[13] = {};
       for (int p = 0; p < nv_; ++p) {
-        const double fv = double(f_[static_cast<i64>(c) * nv_ + p]);
+        double fv = double(f_[static_cast<i64>(c) * nv_ + p]);
+        if (!wqd_.empty()) fv *= wqd_[p];
         const double cx = xd_[p] - u0x, cy = yd_[p] - u0y, cz = zd_[p] - u0z;
         const double f1 = fv * cx, f2 = fv * cy, f3 = fv * cz;
         const double fc2 = fv * (cx * cx + cy * cy + cz * cz);
@@ -478,8 +487,8 @@ class Backend {
       if (deq_) {
         MaxwellParams<F> eq;
         double gam[5];
-        const int r = discrete_maxwell<F>(xx_.data(), xy_.data(), xz_.data(), nv_, w_, g_, m[0], m[1], m[2], m[3], m[4],
-                                          &cp.s, eq, gam);
+        const int r = discrete_maxwell<F>(xx_.data(), xy_.data(), xz_.data(), wq(), nv_, w_, g_, m[0], m[1], m[2], m[3],
+                                          m[4], &cp.s, eq, gam);
         if (r > 0) {
           cp.s.eq = eq;
           cp.s.g0 = F(gam[0]);
@@ -554,6 +563,7 @@ class Backend {
             fb = maxwellian(inflow_[b].m, x, y, z);
           o = 5;
         }
+        if (!wq_.empty()) fb *= wq_[p];
         const F wf = w * fb;
         const F e = F(0.5) * (x * x + y * y + z * z);
         const F af = an * wf;
@@ -594,7 +604,7 @@ class Backend {
           gradient(e.other, p, fj, g);
           ff = fj + g[0] * e.dj[0] + g[1] * e.dj[1] + g[2] * e.dj[2];
         }
-        acc += double(a * ff);
+        acc += wqd_.empty() ? double(a * ff) : double(a * ff) * wqd_[p];
       }
       out[i] = double(F(w_)) * acc;
     }
@@ -623,7 +633,7 @@ class Backend {
         for (int t = 0; t < kc; ++t) {
           const F R = residual(k0, c, t);
           r_[static_cast<i64>(c) * chunk_ + t] = R;
-          r2 += double(R) * double(R);
+          r2 += wqd_.empty() ? double(R) * double(R) : double(R) * double(R) * wqd_[k0 + t];
         }
         rc[c] += r2;
       }
@@ -760,12 +770,14 @@ class Backend {
     if (!deq_) return params_from<F>(rho, ux, uy, uz, T);
     MaxwellParams<F> m;
     double gam[5];
-    const int r = discrete_maxwell<F>(xx_.data(), xy_.data(), xz_.data(), nv_, w_, g_, rho, ux, uy, uz, T, nullptr, m, gam);
+    const int r = discrete_maxwell<F>(xx_.data(), xy_.data(), xz_.data(), wq(), nv_, w_, g_, rho, ux, uy, uz, T, nullptr, m,
+                                      gam);
     if (result) *result = r;
     return m;
   }
 
   F fat(int c, int p) const { return f_[static_cast<i64>(c) * nv_ + p]; }
+  const F* wq() const { return wq_.empty() ? nullptr : wq_.data(); }
   F dotn(const Face<F>& f, int p) const { return xx_[p] * f.n[0] + xy_[p] * f.n[1] + xz_[p] * f.n[2]; }
   F inflow_at(int b, int p) const { return inflow_[b].scale * maxwellian(inflow_[b].m, xx_[p], xy_[p], xz_[p]); }
   F maxwell_inflow(int b, int p) const { return inflow_at(b, p) + F(mw_frac_[b]) * spec_[mw_off_[b] + p]; }
@@ -839,7 +851,19 @@ class Backend {
     const F r[3] = {x - F(2) * un * b.n[0], y - F(2) * un * b.n[1], z - F(2) * un * b.n[2]};
     int base[3];
     F t[3];
+    int node_off = 0;
     for (int a = 0; a < 3; ++a) {
+      if (!nodes_.empty()) {  // stretched grid: bracketing nodes, linear in xi between them
+        const double* nd = nodes_.data() + node_off;
+        const int n = grid_n_[a];
+        node_off += n;
+        const double ra = double(r[a]);
+        if (ra < nd[0] - 1e-5 * (nd[1] - nd[0]) || ra > nd[n - 1] + 1e-5 * (nd[n - 1] - nd[n - 2])) return F(0);
+        const int lo = std::max(0, std::min(n - 2, static_cast<int>(std::upper_bound(nd, nd + n, ra) - nd) - 1));
+        base[a] = lo;
+        t[a] = F(std::max(0.0, std::min(1.0, (ra - nd[lo]) / (nd[lo + 1] - nd[lo]))));
+        continue;
+      }
       const double zz = (double(r[a]) - g_.lo[a]) / g_.d[a] - 0.5;
       if (zz < -1e-5 || zz > grid_n_[a] - 1.0 + 1e-5) return F(0);
       const double cl = std::max(0.0, std::min(zz, grid_n_[a] - 1.0));
@@ -980,6 +1004,8 @@ class Backend {
   std::vector<Face<F>> bf_;
   std::vector<Inflow<F>> inflow_;
   std::vector<CellParams<F>> cells_;
+  std::vector<F> wq_;                 // stretched grid: relative weights (empty: uniform)
+  std::vector<double> wqd_, nodes_;  // the same in FP64; node coordinates of axes x, y, z
 };
 
 // ---- C ABI: one handle = either precision ----

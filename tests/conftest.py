@@ -55,7 +55,7 @@ T0 = 273.15
 
 
 def box_case(mg, directory, n=(5, 3, 2), distort=0.25, seed=7, n_vel=8, vrange=5.0, kinetic=None, bcs="mixed",
-             delta=1.0, name="box", wall_T=(0.95, 1.05)):
+             delta=1.0, name="box", wall_T=(0.95, 1.05), stretch=0.0):
     """Distorted box with every boundary kind the kinetic path handles:
     xmin pressure inlet, xmax pressure outlet (p > 0), ymin moving diffuse wall, ymax Maxwell wall
     (alpha = 0.8, axis aligned -> mirror shortcut), zmin / zmax symmetry."""
@@ -80,7 +80,7 @@ def box_case(mg, directory, n=(5, 3, 2), distort=0.25, seed=7, n_vel=8, vrange=5
         raise ValueError(bcs)
     path = os.path.join(directory, f"{name}.toml")
     mg.write_case(path, f"{name}.pmesh", name, bnd, [n_vel] * 3, [(-vrange, vrange)] * 3,
-                  dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), kinetic)
+                  dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), kinetic, stretch=stretch)
     return path
 
 

@@ -66,6 +66,9 @@ METHOD_CASES = {
                                                          "positivity_limiter": True, "block_passes": 3,
                                                          "velocity_chunk": 64}), 1e-12),
     "box_mixed_fp32": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}), 2e-5),
+    # tanh-stretched velocity grid (an extension; the oracle carries the same per-velocity weights)
+    "box_mixed_fp64_stretched": (dict(kinetic={"precision": "fp64"}, stretch=1.5), 1e-12),
+    "box_mixed_fp32_stretched": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}, stretch=1.5), 2e-5),
 }
 
 
@@ -107,6 +110,11 @@ SOLVER_CASES = {
     "tube_curved_maxwell": lambda mg, d: mg.tube_case(d, n_core=4, n_ring=2, n_axial=3, n_vel=8, vrange=5.0, delta=5.0,
                                                      accommodation=0.9, kinetic={"precision": "fp64"}),
     "slab_200_24cubed": lambda mg, d: mg.slab_case(d, kinetic={"precision": "fp64"}),
+    # tanh-stretched velocity grid; the curved Maxwell wall interpolates between stretched nodes
+    "box_mixed_stretched": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64"}, stretch=1.5),
+    "tube_curved_maxwell_stretched": lambda mg, d: mg.tube_case(d, n_core=4, n_ring=2, n_axial=3, n_vel=8, vrange=5.0,
+                                                               delta=5.0, accommodation=0.9, stretch=1.2,
+                                                               kinetic={"precision": "fp64"}),
 }
 
 
@@ -214,12 +222,15 @@ def _moment_errors(a, b):
     return out
 
 
-def test_mfma_moment_sums_track_fp64_sums(gsk, meshgen, tmp_path):
+@pytest.mark.parametrize("stretch", [0.0, 1.5])
+def test_mfma_moment_sums_track_fp64_sums(gsk, meshgen, tmp_path, stretch):
     """FP32 form: the matrix-core moment contraction (FP32 products and accumulation, FP64 block
-    sums) against the FP64 sums of the same FP32 f, on 24^3 velocities. Stated errors: rho, u, T, p
-    within 2e-6 of their scale, stress deviator and heat flux within 2e-6 of the pressure."""
-    print("[gpu] mfma moment sums", flush=True)
-    path = box_case(meshgen, str(tmp_path), n=(4, 3, 2), n_vel=24, vrange=6.0, kinetic={"precision": "fp32"})
+    sums) against the FP64 sums of the same FP32 f, on 24^3 velocities (uniform and stretched).
+    Stated errors: rho, u, T, p within 2e-6 of their scale, stress deviator and heat flux within 2e-6
+    of the pressure."""
+    print(f"[gpu] mfma moment sums, stretch {stretch}", flush=True)
+    path = box_case(meshgen, str(tmp_path), n=(4, 3, 2), n_vel=24, vrange=6.0, kinetic={"precision": "fp32"},
+                    stretch=stretch)
     ref = gsk.Backend(gsk.Setup(path))
     mma = gsk.Backend(gsk.Setup(path, {"moment_sums": "mfma"}))
     f = perturbed_distribution(ref.setup, seed=5).astype(np.float32).astype(np.float64)
