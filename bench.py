@@ -2,8 +2,10 @@
 
 Workload (default): the 3D tube-to-vacuum case -- O-grid hex cylinder R = L_ref, length 5R,
 ~40k cells (16x16 core + 4x16x10 ring, 45 axial), pressure inlet p_in / T0, vacuum outlet, diffuse
-wall, delta_rp = 10, 24^3 uniform velocity grid on [-6, 6], FP64 (the plain drop-in form), kinetic
-defaults (octant sweeps, 4 inner sweeps with wall update, 2 block passes, discrete equilibrium).
+wall, delta_rp = 10, 24^3 tanh-stretched velocity grid on [-6, 6] (velocity_grid.stretch = 1.5), FP64
+(the plain drop-in form), kinetic defaults (octant sweeps, 4 inner sweeps with wall update, 2 block
+passes, discrete equilibrium). The *_24cubed_fp64 / fp32 / mixed workloads use the reference's
+uniform grid.
 Synthetic mesh from this package's generator; f starts as the rest equilibrium at (p_in, T0).
 
 A step is one CIS iteration (prepare + sweep). One update = one (cell, velocity) pair through one
@@ -40,7 +42,18 @@ def _package():
 
 WORKLOADS = {
     # name: (tube args, velocity points per axis, [kinetic] settings[, case settings]); "mixed" = FP32 f
-    # with the matrix-core moment contraction; case settings: delta (delta_rp, default 10)
+    # with the matrix-core moment contraction; case settings: delta (delta_rp, default 10), stretch
+    # (velocity_grid.stretch, default 0 = the reference's uniform grid); "tanh" = stretch 1.5
+    "tube40k_24cubed_tanh_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp64"}, {"stretch": 1.5}),
+    "tube40k_24cubed_tanh_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32"}, {"stretch": 1.5}),
+    "tube40k_24cubed_tanh_mixed": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32", "moment_sums": "mfma"},
+                                   {"stretch": 1.5}),
+    "tube135k_32cubed_tanh_mixed_delta1": (dict(n_core=24, n_ring=14, n_axial=70), 32,
+                                           {"precision": "fp32", "moment_sums": "mfma"}, {"stretch": 1.5, "delta": 1.0}),
+    "tube135k_32cubed_tanh_mixed_delta100": (dict(n_core=24, n_ring=14, n_axial=70), 32,
+                                             {"precision": "fp32", "moment_sums": "mfma"}, {"stretch": 1.5, "delta": 100.0}),
+    "tube135k_48cubed_tanh_fp32_delta100": (dict(n_core=24, n_ring=14, n_axial=70), 48, {"precision": "fp32"},
+                                            {"stretch": 1.5, "delta": 100.0}),
     "tube40k_24cubed_fp64": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp64"}),
     "tube40k_24cubed_fp32": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32"}),
     "tube40k_24cubed_mixed": (dict(n_core=16, n_ring=10, n_axial=45), 24, {"precision": "fp32", "moment_sums": "mfma"}),
@@ -80,9 +93,11 @@ def measured_traffic(workload, kernel, precision):
     if not os.path.exists(path):
         return None
     data = json.load(open(path))
-    name = f"k_{kernel}<{'double' if precision == 'fp64' else 'float'}>"
-    k = data.get("kernels", {}).get(name)
-    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
+    # every instantiation of the kernel (the sweeps have one per block pass kind), weighted by launches
+    prefix = f"k_{kernel}<{'double' if precision == 'fp64' else 'float'}"
+    ks = [k for name, k in data.get("kernels", {}).items() if name == prefix + ">" or name.startswith(prefix + ",")]
+    n = sum(k["launches"] for k in ks)
+    return None if n == 0 else int(sum((k["fetch_bytes"] + k["write_bytes"]) * k["launches"] for k in ks) / n)
 
 
 def summarize_trace(stats_csv, workload, command):
@@ -133,16 +148,16 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8):
-    """Oracle (CPU restatement, OpenMP) on the same tube mesh with an 8^3 velocity grid (about 10-20 s
-    of CPU work)."""
+def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8, stretch=0.0):
+    """Oracle (CPU restatement, OpenMP) on the same tube mesh with an 8^3 velocity grid of the same
+    kind (about 10-20 s of CPU work)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import gsis_sim_amd.meshgen as mg
 
     threads = oracle.set_threads(min(16, os.cpu_count() or 1))
     path = mg.tube_case(case_dir, n_vel=n_vel, vrange=6.0, delta=10.0, kinetic={"precision": precision}, name="cpu_tube",
-                        **tube_args)
+                        stretch=stretch, **tube_args)
     s = g.Setup(path)
     cb = oracle.CpuBackend(s, "oracle", ops_type=g.BackendOps)
     sv = g.Solver(s, cb.ops)
@@ -158,13 +173,15 @@ def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8):
     updates = s.n_cells * s.nv * inner * iterations
     sv.close()
     return dict(value=updates / dt / 1e9, unit="Gcell·vel updates/s", cores=threads, kind="port",
-                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a {n_vel}^3 velocity grid, "
+                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a {n_vel}^3 "
+                       f"{'tanh-stretched' if stretch > 0 else 'uniform'} velocity grid, "
                        f"{iterations} CIS iterations after 1 untimed ({dt:.1f} s)")
 
 
 def time_to_mdot(g, mg, args):
     """Metric 2: wall time until the inlet Mdot is within 1e-4 relative of the converged FP64 Mdot,
-    for the FP64 form and the FP32 form, on the 40k-cell tube (24^3, delta_rp = 10)."""
+    for the FP64, FP32 and mixed forms, on the 40k-cell tube (24^3 velocities, tanh-stretched unless
+    --uniform-grid, delta_rp = 10)."""
     import gsis_sim_amd.cis_run as cr
 
     tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
@@ -173,7 +190,7 @@ def time_to_mdot(g, mg, args):
         for prec, kin in (("fp64", {"precision": "fp64"}), ("fp32", {"precision": "fp32"}),
                           ("mixed", {"precision": "fp32", "moment_sums": "mfma"})):
             path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=10.0, kinetic=kin, name=f"tube_{prec}",
-                                **tube_args)
+                                stretch=0.0 if args.uniform_grid else 1.5, **tube_args)
             setup = g.Setup(path)
             sv = g.Solver(setup)
             log(f"time-to-Mdot: {prec}")
@@ -182,7 +199,8 @@ def time_to_mdot(g, mg, args):
             sv.close()
             runs[prec] = h
     ref = float(runs["fp64"]["mdot_in"][-1])
-    out = {"metric": "time-to-Mdot (1e-4 rel)", "unit": "s", "workload": "tube40k_24cubed", "mdot_ref_fp64": ref,
+    out = {"metric": "time-to-Mdot (1e-4 rel)", "unit": "s",
+           "workload": "tube40k_24cubed" if args.uniform_grid else "tube40k_24cubed_tanh", "mdot_ref_fp64": ref,
            "residual_target": args.residual_target}
     for prec, h in runs.items():
         t = cr.time_to_within(h, ref, 1e-4)
@@ -208,8 +226,11 @@ def sensitivity_bench(g, mg, args):
     delta, alpha, h = 10.0, args.alpha, args.fd_step
     tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
     with tempfile.TemporaryDirectory() as d:
-        path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=delta, accommodation=alpha, kinetic={"precision": "fp64"},
-                            name="tube_maxwell", **tube_args)
+        # alpha = 1 through the Maxwell wall's smooth extension: the case gets a Maxwell wall (alpha < 1;
+        # the reference turns alpha = 1 into a diffuse wall) and the solve then sets alpha, so central
+        # differences may step past 1 (specular fraction 1 - alpha < 0)
+        path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=delta, accommodation=min(alpha, 0.99),
+                            kinetic={"precision": "fp64"}, name="tube_maxwell", **tube_args)
         setup = g.Setup(path)
         sv = g.Solver(setup)
         prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
@@ -224,7 +245,7 @@ def sensitivity_bench(g, mg, args):
         log(f"forward: {fwd_its} iterations, {fwd_s:.1f} s, Mdot {float(mdot.detach()):.10f}; sensitivity solve")
         prob.polish = args.polish
         mdot.backward()
-        out = {"metric": "sensitivity / forward cost", "workload": "tube40k_24cubed_fp64_maxwell",
+        out = {"metric": "sensitivity / forward cost", "workload": f"tube40k_24cubed_fp64_maxwell_alpha{alpha:g}",
                "delta_rp": delta, "alpha": alpha, "mdot": float(mdot.detach()), "forward_seconds": fwd_s,
                "forward_iterations": fwd_its, "residual_target": args.sens_residual,
                "sensitivity_seconds": prob.stats["sensitivity_seconds"],
@@ -254,7 +275,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="tube40k_24cubed_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="tube40k_24cubed_tanh_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--uniform-grid", action="store_true", help="time-to-Mdot on the reference's uniform velocity grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-to-mdot", action="store_true",
                     help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
@@ -318,11 +340,12 @@ def main():
     tube_args, nvel, kinetic = WORKLOADS[args.workload][:3]
     case = WORKLOADS[args.workload][3] if len(WORKLOADS[args.workload]) > 3 else {}
     delta = float(case.get("delta", 10.0))
+    stretch = float(case.get("stretch", 0.0))
     precision = kinetic["precision"]
     s_bytes = 8 if precision == "fp64" else 4
     with tempfile.TemporaryDirectory() as d:
         log(f"generating {args.workload}")
-        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=delta, kinetic=dict(kinetic), **tube_args)
+        path = mg.tube_case(d, n_vel=nvel, vrange=6.0, delta=delta, kinetic=dict(kinetic), stretch=stretch, **tube_args)
         setup = g.Setup(path)
         log(f"setup: {setup.n_cells} cells, {setup.nv} velocities, {setup.n_blocks} blocks")
         sv = g.Solver(setup)
@@ -374,7 +397,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": args.workload, "cells": setup.n_cells, "velocities": setup.nv,
                        "inner_sweeps": inner, "block_passes": passes, "sweep_blocks": setup.n_blocks,
-                       "chunk_width": chunk, "delta_rp": delta},
+                       "chunk_width": chunk, "delta_rp": delta,
+                       "velocity_grid": f"tanh-stretched (stretch {stretch}) on [-6, 6]^3" if stretch > 0 else "uniform on [-6, 6]^3"},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": measured_traffic(args.workload, dom, precision),
@@ -387,7 +411,7 @@ def main():
         }
         if not args.no_cpu_baseline:
             log("cpu baseline")
-            result["cpu_baseline"] = cpu_baseline(g, d, tube_args, precision)
+            result["cpu_baseline"] = cpu_baseline(g, d, tube_args, precision, stretch=stretch)
         else:
             result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

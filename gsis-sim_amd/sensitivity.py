@@ -290,8 +290,8 @@ class SteadyProblem:
 
 class ConvergedMdot(torch.autograd.Function):
     """Mdot of the converged kinetic solve as a function of (delta_rp, alpha); backward returns the
-    sensitivities from SteadyProblem.sensitivities (FP64 difference Jacobian, LU-SGS-preconditioned
-    GMRES)."""
+    sensitivities from SteadyProblem.sensitivities (difference Jacobian-vector products, LU-SGS right
+    preconditioning, BiCGStab with optional restarted-GMRES polishing)."""
 
     @staticmethod
     def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):

@@ -16,6 +16,15 @@
 //
 // Input is the same gsk_setup_view the HIP backend takes; it exports the gsk_backend_ops method
 // set so the library's solver orchestration can drive it (gsk_solver_create_with_backend).
+//
+// Extension beyond the reference: the tanh-stretched velocity grid (gsk_setup_view.weight_rel /
+// grid_nodes non-NULL). Every velocity sum then weighs term p by weight_rel[p], and the Maxwell-wall
+// interpolation brackets the reflected velocity between stretched nodes. The reference has no such
+// grid, so these paths are pinned by closed forms instead (tests/test_stretched_grid.py: Gaussian
+// quadrature of the weights, Maxwellian moments to 1e-10, exact conservation of the discrete
+// equilibrium, equilibrium preservation). The Maxwell-wall interpolation on a stretched grid has no
+// closed-form check: parity unpinned (the GPU matches it, which pins only their agreement). On the
+// uniform grid both pointers are NULL and every path is the reference's, bit for bit.
 
 #include <omp.h>
 
