@@ -165,8 +165,11 @@ class SteadyProblem:
             if rel <= tol:
                 break
             m = min(restart, max_iterations - total)
-            V = torch.empty((m + 1, self.n), dtype=self.dtype, device="cuda")
-            V[0] = r / beta
+            # the basis as separate vectors: each reuses a cached block of the solve's own vector size,
+            # where one (m + 1) x n tensor would need a contiguous m + 1 vectors' worth (129 GiB at
+            # the 40k-cell tube with m = 30) that a fragmented cache may not have
+            torch.cuda.empty_cache()
+            V = [r / beta]
             H = np.zeros((m + 1, m))
             cs, sn = np.zeros(m), np.zeros(m)
             g = np.zeros(m + 1)
@@ -175,12 +178,12 @@ class SteadyProblem:
             for k in range(m):
                 w, _ = self._jv(self._precondition(V[k]))
                 for _ in range(2):  # classical Gram-Schmidt, twice
-                    h = V[: k + 1] @ w
-                    w = w - h @ V[: k + 1]
-                    H[: k + 1, k] += h.double().cpu().numpy()
+                    h = torch.stack([torch.dot(V[i], w) for i in range(k + 1)]).double().cpu().numpy()
+                    for i in range(k + 1):
+                        w.add_(V[i], alpha=-float(h[i]))
+                    H[: k + 1, k] += h
                 H[k + 1, k] = float(torch.linalg.vector_norm(w))
-                if H[k + 1, k] > 0.0:
-                    V[k + 1] = w / H[k + 1, k]
+                V.append(w / H[k + 1, k] if H[k + 1, k] > 0.0 else torch.zeros_like(w))
                 for i in range(k):  # apply the previous Givens rotations
                     t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
                     H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]
@@ -199,9 +202,11 @@ class SteadyProblem:
                 if rel <= tol:
                     break
             y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
-            yt = torch.as_tensor(y, dtype=self.dtype, device="cuda")
-            s = s + self._precondition(yt @ V[:k_used])
-            del V
+            z = torch.zeros_like(s)
+            for i in range(k_used):
+                z.add_(V[i], alpha=float(y[i]))
+            s = s + self._precondition(z)
+            del V, z
             Js, _ = self._jv(s)
             r = b - Js
         return s, total, rel
