@@ -234,7 +234,7 @@ def sensitivity_bench(g, mg, args):
         setup = g.Setup(path)
         sv = g.Solver(setup)
         prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
-                                  alpha=alpha)
+                                  alpha=alpha, eps=args.sens_eps)
         prob.log = lambda msg: log(msg)
         d_t = torch.tensor(delta, dtype=torch.float64, requires_grad=True)
         a_t = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
@@ -246,7 +246,8 @@ def sensitivity_bench(g, mg, args):
         prob.polish = args.polish
         mdot.backward()
         out = {"metric": "sensitivity / forward cost", "workload": f"tube40k_24cubed_fp64_maxwell_alpha{alpha:g}",
-               "delta_rp": delta, "alpha": alpha, "mdot": float(mdot.detach()), "forward_seconds": fwd_s,
+               "delta_rp": delta, "alpha": alpha, "sens_eps": args.sens_eps, "mdot": float(mdot.detach()),
+               "forward_seconds": fwd_s,
                "forward_iterations": fwd_its, "residual_target": args.sens_residual,
                "sensitivity_seconds": prob.stats["sensitivity_seconds"],
                "cost_ratio": prob.stats["sensitivity_seconds"] / fwd_s,
@@ -290,6 +291,8 @@ def main():
     ap.add_argument("--fd-check", action="store_true", help="with --sensitivity: central differences (4 solves)")
     ap.add_argument("--fd-residual", type=float, default=1e-10, help="residual target of the difference solves")
     ap.add_argument("--polish", type=int, default=120, help="GMRES iterations polishing the BiCGStab solution")
+    ap.add_argument("--sens-eps", type=float, default=1e-7,
+                    help="relative perturbation of the difference Jacobian-vector products")
     ap.add_argument("--fd-params", default="delta_rp,alpha", help="parameters the differences check")
     ap.add_argument("--fd-step", type=float, default=1e-3,
                     help="central-difference step (relative for delta_rp, absolute for alpha)")

@@ -7,11 +7,15 @@ At the converged state f* the steady residual vanishes, R(f*; p) = 0, so
 R is the solver's full steady residual (transport with the limited reconstruction, Shakhov
 collision with the discrete equilibrium, every boundary condition, wall densities and inflow states
 recomputed from f), evaluated on the device through gsk_solver_residual_device. J s = -dR/dp is
-solved by restarted GMRES with right preconditioning by the LU-SGS passes of one inner sweep
-(gsk_solver_precondition_device, the same level-scheduled kernels as the forward solve), with
-Jacobian-vector products as central FP64 differences of R (relative step eps = 1e-5, truncation
-error ~eps^2). The linear system is ill-conditioned (as slow as the CIS iteration itself), so the
-products must be accurate to ~1e-9, and R must be a function of f alone. Two settings make it so
+solved by BiCGStab (then restarted GMRES on the true residual) with right preconditioning by the
+LU-SGS passes of one inner sweep (gsk_solver_precondition_device, the same level-scheduled kernels
+as the forward solve), with Jacobian-vector products as central FP64 differences of R. The step is a
+relative eps = 1e-7 of |f*|: R is only piecewise smooth (the limiter's min / max and clamp), and a
+larger step (1e-5 measured) crosses enough kinks that the difference products stop being linear in
+the direction, which leaves the true linear residual at 5-11% and the sensitivities off by 1-2e-4;
+at 1e-7 the rounding noise of the differences is still ~1e-9 of the signal. The linear system is
+ill-conditioned (as slow as the CIS iteration itself), so the products must be accurate to ~1e-9,
+and R must be a function of f alone. Two settings make it so
 for the duration of the sensitivity solve (restored afterwards): the discrete-equilibrium Newton
 solves inside R (cells and inflow states) run until the Newton step stops improving instead of
 stopping at a tolerance, and every prepare runs 10 inflow / boundary-sum rounds, because the
@@ -41,7 +45,7 @@ class SteadyProblem:
     parameter (None if alpha is not needed)."""
 
     def __init__(self, solver, inlet: int, mdot_factor: float, wall_patch: int | None = None,
-                 delta_rp: float | None = None, alpha: float | None = None, eps: float = 1e-5):
+                 delta_rp: float | None = None, alpha: float | None = None, eps: float = 1e-7):
         self.sv = solver
         self.inlet = inlet
         self.factor = mdot_factor
