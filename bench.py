@@ -105,7 +105,7 @@ def summarize_trace(stats_csv, workload, command):
     import csv
 
     rows = list(csv.DictReader(open(stats_csv)))
-    lines = [f"# rocprofv3 kernel trace: {workload}", "", "Command (repo root on the GPU box):", "",
+    lines = [f"# rocprofv3 kernel trace: {workload}", "", "Command (run from the repository root):", "",
              f"    {command}", "", "| kernel | calls | total ms | mean ms | % |", "|---|---|---|---|---|"]
     for r in rows:
         name = r["Name"].split("(")[0].replace("void ", "").replace("gsk::dev::", "")

@@ -373,7 +373,7 @@ CaseConfig parse_case_toml(const std::string& text, const std::string& source) {
     Reader r{doc.tables.at("velocity_grid"), "velocity_grid"};
     const TomlValue* aut = r.find("auto");
     GSK_CHECK(aut == nullptr || (aut->kind == TomlValue::Bool && !aut->b),
-              "velocity_grid.auto is not supported by this build; give n and range");
+              "velocity_grid.auto is not supported; give n and range");
     const TomlValue* n = r.find("n");
     const TomlValue* range = r.find("range");
     GSK_CHECK(n && n->kind == TomlValue::Array && n->arr.size() == 3, "velocity_grid.n must be [nx, ny, nz]");

@@ -278,7 +278,7 @@ HELIUM = dict(molecular_mass=6.65e-27, mu_ref=1.916e-5, T_ref=273.15, omega=0.66
 def write_case(path: str, mesh_file: str, name: str, boundaries: list, grid_n, grid_range, conditions: dict,
                kinetic: dict | None = None, gas: dict | None = None, stretch: float = 0.0) -> None:
     """Write a case TOML in the reference's schema (src/config/case_config.cpp:312-460); stretch > 0 adds
-    velocity_grid.stretch, this build's tanh-stretched velocity grid (no reference counterpart)."""
+    velocity_grid.stretch, the tanh-stretched velocity grid (an extension with no reference counterpart)."""
     g = dict(HELIUM, **(gas or {}))
     lines = ["[case]", f'name = "{name}"', f'mesh = "{mesh_file}"', 'mode = "cis"', "", "[gas]"]
     lines += [f"{k} = {v!r}" for k, v in g.items()]

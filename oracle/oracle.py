@@ -2,7 +2,7 @@
 
 Loads oracle/liboracle.so (the CPU restatement of the reference's CPU backend) and, when built,
 oracle/_ref/libref_kernels.so (the reference's own kernel headers compiled in place). Both expose
-the gsk_backend_ops table, so the library's solver orchestration can drive them as the checker.
+the gsk_backend_ops table, so the library's solver orchestration can drive them.
 """
 
 from __future__ import annotations
@@ -19,7 +19,7 @@ _libs: dict = {}
 
 
 def build(reference: str = "/root/reference") -> None:
-    """Compile the oracle (and the reference-kernel checker when the reference tree exists)."""
+    """Compile the oracle (and oracle/_ref, the reference's own kernels, when the reference tree exists)."""
     subprocess.run(["make", "-s", "-C", HERE, f"REFERENCE={reference}"], check=True)
 
 
@@ -45,7 +45,7 @@ def ref_available() -> bool:
 
 
 class CpuBackend:
-    """A checker backend: kind = "oracle" (restatement) or "ref" (reference kernels)."""
+    """A CPU backend for the tests: kind = "oracle" (restatement) or "ref" (reference kernels)."""
 
     def __init__(self, setup, kind: str = "oracle", precision: int = -1, ops_type=None):
         path, prefix = (ORACLE_LIB, "orc") if kind == "oracle" else (REF_LIB, "ref")

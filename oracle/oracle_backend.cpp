@@ -1,5 +1,5 @@
 // ORACLE -- test infrastructure only. Nothing in the shipped library links, loads or calls this
-// file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the checker.
+// file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, to check results.
 //
 // A CPU restatement (C++17 + OpenMP) of the reference's CPU kinetic backend, the oracle the HIP
 // backend is compared with. Paths are relative to the reference tree; each function names the

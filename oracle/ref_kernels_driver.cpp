@@ -1,4 +1,4 @@
-// CHECKER ONLY -- builds oracle/_ref/libref_kernels.so from the reference's own per-point kernel
+// TEST INFRASTRUCTURE ONLY -- builds oracle/_ref/libref_kernels.so from the reference's own per-point kernel
 // headers, which are compiled here straight from the reference tree (-I <reference>/src; nothing is
 // copied). Used only by tests/ to pin the oracle; never shipped, never loaded by the library.
 //

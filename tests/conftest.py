@@ -1,4 +1,4 @@
-"""Shared fixtures: the package (loaded from gsis-sim_amd/), the checker backends (oracle/), and
+"""Shared fixtures: the package (loaded from gsis-sim_amd/), the oracle backends (oracle/), and
 small programmatic cases in the style of the reference's make_kinetic_case / make_box_mesh helpers
 (tests/kinetic/kinetic_test_cases.cpp:9-60, tests/support/box_mesh.cpp of the reference)."""
 

@@ -3,8 +3,8 @@
 Maxwellian, 300 CIS iterations).
 
 The expected values come from the CPU oracle (oracle/liboracle.so) driven by the library's solver
-orchestration; when the reference-kernel checker (oracle/_ref, built from the reference tree in this
-container) is present, the same run is repeated with it and the two must agree bit for bit before
+orchestration; when the reference's own kernels (oracle/_ref, built from the reference tree where it
+exists) is present, the same run is repeated with it and the two must agree bit for bit before
 anything is written. Output: slab_c1.npz with moments after iterations 1, 10 and 300 (original cell
 numbering), the residual / epsilon history, and the wall fluxes at the end.
 

@@ -20,14 +20,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def _pair(gsk, oracle_mod, setup, precision=-1):
     if not oracle_mod.ref_available():
-        pytest.skip("reference-kernel checker (oracle/_ref) not built: reference tree absent")
+        pytest.skip("reference kernels (oracle/_ref) not built: reference tree absent")
     a = oracle_mod.CpuBackend(setup, "oracle", precision, ops_type=gsk.BackendOps)
     b = oracle_mod.CpuBackend(setup, "ref", precision, ops_type=gsk.BackendOps)
     return a, b
 
 
 class OpsCaller:
-    """Call a gsk_backend_ops table from Python (used on the checker backends)."""
+    """Call a gsk_backend_ops table from Python (used on the oracle backends)."""
 
     def __init__(self, cb, setup):
         import ctypes as C
