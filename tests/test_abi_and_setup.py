@@ -146,3 +146,38 @@ def test_generated_slab_matches_reference_mesh(gsk, meshgen, tmp_path):
     va = np.ctypeslib.as_array(a.view().volume, (200,))
     vb = np.ctypeslib.as_array(b.view().volume, (200,))
     assert np.allclose(va, vb, rtol=1e-12)
+
+
+def test_solver_parameter_setters_validate(gsk, meshgen, oracle_mod, tmp_path):
+    """The solver-level setters used by the sensitivity solve reject what they cannot honour, and the
+    device-pointer entry points fail loudly on a backend without device buffers (the oracle)."""
+    setup = gsk.Setup(box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"}))
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+    sv = gsk.Solver(setup, cb.ops)
+    with pytest.raises(gsk.GskError, match="Maxwell walls only"):
+        sv.set_accommodation(0, 0.9)  # patch 0 is the pressure inlet
+    with pytest.raises(gsk.GskError, match="positive"):
+        sv.set_accommodation(3, 0.0)  # patch 3 is the Maxwell wall
+    with pytest.raises(gsk.GskError, match="boundary iterations"):
+        sv.set_boundary_iterations(0)
+    with pytest.raises(gsk.GskError, match="Newton tolerances"):
+        sv.set_newton_tolerances(-1.0, 0.0)
+    sv.set_boundary_iterations(3)
+    assert sv.mu0() > 0.0
+    # what needs the backend's device-resident operators fails loudly on a backend without them
+    for call in (lambda: sv.set_accommodation(3, 0.7), lambda: sv.set_mu0(2.0 * sv.mu0())):
+        with pytest.raises(gsk.GskError, match="no device-resident"):
+            call()
+    with pytest.raises(gsk.GskError, match="null argument"):
+        sv.residual_device(0, setup.n_cells * setup.nv)
+    sv.close()
+
+
+def test_velocity_grid_stretch_is_validated(gsk, meshgen, tmp_path):
+    path = box_case(meshgen, str(tmp_path), stretch=1.5)
+    text = open(path).read()
+    assert "stretch = 1.5" in text
+    bad = os.path.join(str(tmp_path), "bad_stretch.toml")
+    open(bad, "w").write(text.replace("stretch = 1.5", "stretch = -1.0"))
+    with pytest.raises(gsk.GskError, match="stretch must be in"):
+        gsk.Setup(bad)
