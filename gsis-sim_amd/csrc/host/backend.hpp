@@ -26,6 +26,9 @@ class Backend {
   virtual void set_inflow(const gsk_inflow_state* inflow, std::size_t n) = 0;
   virtual void boundary_sums(double* out, std::size_t n) = 0;
   virtual void sweep(double* residual_cell, std::size_t n) = 0;
+  // A sweep whose per-cell residual sums the caller does not read (the inner sweeps after the
+  // first); a backend may then skip returning them. Default: sweep() into the scratch span.
+  virtual void sweep_unreported(double* scratch, std::size_t n) { sweep(scratch, n); }
   virtual long long gsis_correction(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
                                     std::size_t n_flags) = 0;
   virtual void internal_face_mass_flux(const int* cells, const int* entries, std::size_t n, double* out) = 0;
@@ -45,6 +48,27 @@ class Backend {
     (void)states;
     (void)n;
     return false;
+  }
+  // The wall part of the refresh between inner sweeps (KineticSolver::refresh_wall_inflow,
+  // kinetic_solver.cpp:456-480) kept on the device: boundary sums of the listed wall faces, then
+  // their inflow scales from those sums (maxwell[i] != 0: Maxwell wall with unit-density sum
+  // unit0[i]; else diffuse wall with the given relaxation), starting from the scales of the last
+  // set_inflow. Returns false when the backend does not provide it; the solver then refreshes on
+  // the host. wall_inflow_scales brings the scales of the listed faces back and raises the error a
+  // negative Maxwell-wall density would have raised during the refresh.
+  virtual bool refresh_wall_inflow(const int* faces, const int* maxwell, const double* unit0, std::size_t n,
+                                   double relaxation) {
+    (void)faces;
+    (void)maxwell;
+    (void)unit0;
+    (void)n;
+    (void)relaxation;
+    return false;
+  }
+  virtual void wall_inflow_scales(const int* faces, std::size_t n, double* out) {
+    (void)faces;
+    (void)n;
+    (void)out;
   }
 
   // Device-resident operators for sensitivity solves (no counterpart in the reference backend).

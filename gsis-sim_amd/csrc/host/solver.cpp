@@ -447,6 +447,20 @@ void KineticSolver::refresh_wall_inflow() {  // (:456-480)
     }
   }
   if (wall_faces_.empty()) return;
+  // on the device when the backend keeps the scales there (no host round trip per inner sweep)
+  wall_maxwell_.resize(wall_faces_.size());
+  wall_unit0_.resize(wall_faces_.size());
+  for (std::size_t i = 0; i < wall_faces_.size(); ++i) {
+    const auto bi = static_cast<std::size_t>(wall_faces_[i]);
+    const bool mw = patches_[static_cast<std::size_t>(s_.bface_patch[bi])].kind == kBcMaxwellWall;
+    wall_maxwell_[i] = mw ? 1 : 0;
+    wall_unit0_[i] = mw ? wall_unit_moments_[bi][0] : 0.0;
+  }
+  if (backend_->refresh_wall_inflow(wall_faces_.data(), wall_maxwell_.data(), wall_unit0_.data(), wall_faces_.size(),
+                                    s_.params.wall_relaxation)) {
+    wall_scales_on_device_ = true;
+    return;
+  }
   backend_->boundary_sums_subset(wall_faces_.data(), wall_faces_.size(), bsums_.data(), bsums_.size());
   bool any = false;
   for (Index b = 0; b < s_.n_bfaces(); ++b) {
@@ -503,7 +517,13 @@ const gsk_iteration_info& KineticSolver::sweep(bool update_pressure) {  // (:503
   for (long long m = 1; m < s_.params.inner_sweeps; ++m) {
     if (s_.params.inner_wall_update) refresh_wall_inflow();
     inner_residual_.resize(residual_cell_.size());
-    backend_->sweep(inner_residual_.data(), inner_residual_.size());
+    backend_->sweep_unreported(inner_residual_.data(), inner_residual_.size());
+  }
+  if (wall_scales_on_device_) {  // the host copy of the wall scales catches up once per iteration
+    wall_scales_on_device_ = false;
+    std::vector<double> sc(wall_faces_.size());
+    backend_->wall_inflow_scales(wall_faces_.data(), wall_faces_.size(), sc.data());
+    for (std::size_t i = 0; i < wall_faces_.size(); ++i) inflow_[static_cast<std::size_t>(wall_faces_[i])].scale = sc[i];
   }
   double sum = 0.0, vol = 0.0;
   for (Index c = 0; c < s_.n_cells(); ++c) {
@@ -517,8 +537,8 @@ const gsk_iteration_info& KineticSolver::sweep(bool update_pressure) {  // (:503
   has_prev_ = true;
   ++version_;
   ++iteration_;
+  info_.timings = backend_->take_timings();  // waits for the backend's queued work
   info_.seconds += now_seconds() - t0;
-  info_.timings = backend_->take_timings();
   return info_;
 }
 

@@ -89,7 +89,9 @@ class KineticSolver {
   std::vector<double> residual_cell_, inner_residual_;
   double mass_target_ = 0.0;
   gsk_inflow_state wall_cache_{};
-  std::vector<int> wall_faces_;
+  std::vector<int> wall_faces_, wall_maxwell_;
+  std::vector<double> wall_unit0_;
+  bool wall_scales_on_device_ = false;  // inflow_ scales of wall faces are behind the backend's
   int wall_cache_patch_ = -1;
   bool has_mass_target_ = false;
 };

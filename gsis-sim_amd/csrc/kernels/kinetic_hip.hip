@@ -508,6 +508,35 @@ __global__ void k_boundary_reduce(const int* __restrict__ faces, int n_faces, in
   out[static_cast<i64>(b) * kBoundarySums + k] = s;
 }
 
+// Wall part of the inner-sweep boundary refresh (the reference's KineticSolver::refresh_wall_inflow,
+// src/kinetic/kinetic_solver.cpp:456-480) from the wall faces' boundary sums s: a Maxwell wall takes
+// scale <- max(0, scale - (s[0] + s[5]) / unit0) and flags a value below -1e-8; a diffuse wall takes
+// rho = max(0, -s[0] / s[5]) (0 unless s[5] < 0) and scale <- rho, or w rho + (1 - w) scale when
+// w = relaxation < 1. The FP64 scales are the solver's own; inflow[b].scale is their cast to F.
+template <class F>
+__global__ void k_wall_refresh(const int* __restrict__ faces, const int* __restrict__ maxwell,
+                               const double* __restrict__ unit0, int n, double relaxation,
+                               const double* __restrict__ bsums, double* __restrict__ scale,
+                               Inflow<F>* __restrict__ inflow, int* __restrict__ negative) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = faces[i];
+  const double* s = bsums + static_cast<i64>(b) * kBoundarySums;
+  double sc = scale[b];
+  if (maxwell[i]) {
+    const double next = sc - (s[0] + s[5]) / unit0[i];
+    if (next < -1e-8) *negative = 1;
+    sc = 0.0 < next ? next : 0.0;
+  } else {
+    const double jp = s[0], jm = s[5];
+    const double q = jm < 0.0 ? -jp / jm : 0.0;
+    const double rho = jm < 0.0 ? (0.0 < q ? q : 0.0) : 0.0;
+    sc = relaxation == 1.0 ? rho : relaxation * rho + (1.0 - relaxation) * sc;
+  }
+  scale[b] = sc;
+  inflow[b].scale = static_cast<F>(sc);
+}
+
 // section mass flux through listed internal faces (k_internal_face_mass_flux)
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_face_mass_flux(View<F> v, const int* __restrict__ cells,
@@ -1284,6 +1313,10 @@ class HipBackend final : public Backend {
     host_bface_kind_.resize(static_cast<std::size_t>(nb_));
     for (int b = 0; b < nb_; ++b) host_bface_kind_[static_cast<std::size_t>(b)] = s.bfaces[b].kind;
     inflow_.alloc(static_cast<std::size_t>(nb_));
+    wall_scale_.alloc(static_cast<std::size_t>(nb_));
+    wall_negative_.alloc(1);
+    HIPCHK(hipMemsetAsync(wall_negative_.p, 0, sizeof(int), stream_));
+    if (nb_ > 0) HIPCHK(hipMemsetAsync(wall_scale_.p, 0, wall_scale_.bytes(), stream_));
     HIPCHK(hipMemsetAsync(inflow_.p, 0, inflow_.bytes(), stream_));
     block_off_.upload(s.block_offsets, static_cast<std::size_t>(n_blocks_) + 1, stream_);
     n_sweep_dirs_ = s.n_sweep_dirs;
@@ -1527,12 +1560,60 @@ class HipBackend final : public Backend {
   void set_inflow(const gsk_inflow_state* in, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nb_), "inflow length differs from the boundary-face count");
     std::vector<dev::Inflow<F>> h(n);
+    wall_scale_host_.resize(n);
     for (std::size_t b = 0; b < n; ++b) {
       h[b].m = dev::maxw_from_state<F>(in[b].rho, in[b].ux, in[b].uy, in[b].uz, in[b].T);
       h[b].scale = static_cast<F>(in[b].scale);
+      wall_scale_host_[b] = in[b].scale;
     }
-    if (n) HIPCHK(hipMemcpyAsync(inflow_.p, h.data(), n * sizeof(dev::Inflow<F>), hipMemcpyHostToDevice, stream_));
+    if (n) {
+      HIPCHK(hipMemcpyAsync(inflow_.p, h.data(), n * sizeof(dev::Inflow<F>), hipMemcpyHostToDevice, stream_));
+      HIPCHK(hipMemcpyAsync(wall_scale_.p, wall_scale_host_.data(), n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    }
     HIPCHK(hipStreamSynchronize(stream_));
+  }
+
+  // The wall refresh between inner sweeps stays on the device: the listed faces' sums, then
+  // k_wall_refresh updates the FP64 scales (kept beside inflow_, loaded by set_inflow) and
+  // inflow_'s F scales in place. Nothing returns to the host until wall_inflow_scales.
+  bool refresh_wall_inflow(const int* faces, const int* maxwell, const double* unit0, std::size_t n,
+                           double relaxation) override {
+    if (n == 0) return true;
+    GSK_CHECK(n <= static_cast<std::size_t>(nb_), "more wall faces than boundary faces");
+    if (wall_faces_host_.size() != n || !std::equal(faces, faces + n, wall_faces_host_.begin()) ||
+        !std::equal(maxwell, maxwell + n, wall_kind_host_.begin()) ||
+        !std::equal(unit0, unit0 + n, wall_unit0_host_.begin())) {
+      for (std::size_t i = 0; i < n; ++i) GSK_CHECK(faces[i] >= 0 && faces[i] < nb_, "face index out of range");
+      wall_faces_host_.assign(faces, faces + n);
+      wall_kind_host_.assign(maxwell, maxwell + n);
+      wall_unit0_host_.assign(unit0, unit0 + n);
+      wall_faces_.upload(wall_faces_host_.data(), n, stream_);
+      wall_kind_.upload(wall_kind_host_.data(), n, stream_);
+      wall_unit0_.upload(wall_unit0_host_.data(), n, stream_);
+    }
+    begin(kBoundary);
+    launch_subset_sums(wall_faces_.p, n);
+    dev::k_wall_refresh<F><<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream_>>>(
+        wall_faces_.p, wall_kind_.p, wall_unit0_.p, static_cast<int>(n), relaxation, bsums_.p, wall_scale_.p, inflow_.p,
+        wall_negative_.p);
+    HIPCHK(hipGetLastError());
+    end();
+    return true;
+  }
+
+  void wall_inflow_scales(const int* faces, std::size_t n, double* out) override {
+    download(wall_scale_host_.data(), wall_scale_.p, wall_scale_host_.size());
+    for (std::size_t i = 0; i < n; ++i) {
+      GSK_CHECK(faces[i] >= 0 && faces[i] < nb_, "face index out of range");
+      out[i] = wall_scale_host_[static_cast<std::size_t>(faces[i])];
+    }
+    int negative = 0;
+    download(&negative, wall_negative_.p, 1);
+    if (negative) {
+      HIPCHK(hipMemsetAsync(wall_negative_.p, 0, sizeof(int), stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+      throw Error("Maxwell wall diffuse density became negative; refine the velocity grid");
+    }
   }
 
   bool correct_inflow(gsk_inflow_state* states, std::size_t n) override {
@@ -1566,20 +1647,25 @@ class HipBackend final : public Backend {
     GSK_CHECK(n == static_cast<std::size_t>(nb_) * dev::kBoundarySums, "boundary sums length differs from 20 x faces");
     if (n_faces == 0) return;
     for (std::size_t i = 0; i < n_faces; ++i) GSK_CHECK(faces[i] >= 0 && faces[i] < nb_, "face index out of range");
+    if (subset_host_.size() != n_faces || !std::equal(faces, faces + n_faces, subset_host_.begin())) {
+      subset_host_.assign(faces, faces + n_faces);
+      subset_.upload(subset_host_.data(), n_faces, stream_);
+    }
     begin(kBoundary);
+    launch_subset_sums(subset_.p, n_faces);
+    HIPCHK(hipGetLastError());
+    end();
+    download(out, bsums_.p, n);
+  }
+
+  // Maxwell-wall reflection caches, then the boundary sums of the listed faces (device list faces_dev)
+  void launch_subset_sums(const int* faces_dev, std::size_t n_faces) {
     if (n_mw_ > 0) {
       const dim3 grid((nv_ + 255) / 256, nb_);
       dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
       dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
     }
-    if (subset_host_.size() != n_faces || !std::equal(faces, faces + n_faces, subset_host_.begin())) {
-      subset_host_.assign(faces, faces + n_faces);
-      subset_.upload(faces, n_faces, stream_);
-    }
-    launch_boundary_sums(subset_.p, static_cast<int>(n_faces));
-    HIPCHK(hipGetLastError());
-    end();
-    download(out, bsums_.p, n);
+    launch_boundary_sums(faces_dev, static_cast<int>(n_faces));
   }
 
   // partial sums over velocity chunks, then their fixed-order sum into bsums_
@@ -1677,6 +1763,18 @@ class HipBackend final : public Backend {
 
   void sweep(double* residual_cell, std::size_t n) override {
     GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
+    launch_sweep();
+    download(residual_cell, residual_.p, n);
+  }
+
+  // no download and no stream synchronisation: the next inner sweep queues behind this one
+  void sweep_unreported(double* scratch, std::size_t n) override {
+    (void)scratch;
+    GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
+    launch_sweep();
+  }
+
+  void launch_sweep() {
     HIPCHK(hipMemsetAsync(residual_.p, 0, residual_.bytes(), stream_));
     const dev::View<F> v = view();
     for (int k0 = 0; k0 < nv_; k0 += chunk_) {
@@ -1684,7 +1782,6 @@ class HipBackend final : public Backend {
       chunk_residual(v, ch);
       chunk_passes(v, ch, true);
     }
-    download(residual_cell, residual_.p, n);
   }
 
   // ---- device-resident operators for sensitivity solves (arrays [cell][storage position], type F) ----
@@ -2100,6 +2197,10 @@ class HipBackend final : public Backend {
   DBuf<dev::SweepRec<F>> fw_recs_, bw_recs_;
   DBuf<int> subset_;
   std::vector<int> subset_host_;
+  DBuf<int> wall_faces_, wall_kind_, wall_negative_;  // device wall refresh: faces, Maxwell flags, error flag
+  DBuf<double> wall_unit0_, wall_scale_;             // unit-density sums s[0] of Maxwell walls; FP64 scales
+  std::vector<int> wall_faces_host_, wall_kind_host_;
+  std::vector<double> wall_unit0_host_, wall_scale_host_;
   DBuf<double> bsum_part_, inflow_stage_;
   int bsum_chunks_ = 1;
   std::vector<int> vel_list_host_, dir_off_;

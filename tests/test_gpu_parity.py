@@ -106,6 +106,10 @@ def _run_solver(gsk, setup, ops, iterations, snapshot_every=None):
 SOLVER_CASES = {
     "box_mixed": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64"}),
     "box_closed": lambda mg, d: box_case(mg, d, bcs="closed", kinetic={"precision": "fp64"}),
+    # relaxed diffuse-wall density between inner sweeps (the HIP solver refreshes walls on the
+    # device, the oracle on the host)
+    "box_mixed_wall_relaxation": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64", "wall_relaxation": 0.6,
+                                                                        "inner_sweeps": 3}),
     "box_vacuum_3blocks": lambda mg, d: box_case(mg, d, bcs="vacuum", kinetic={"precision": "fp64", "sweep_blocks": 3}),
     "tube_curved_maxwell": lambda mg, d: mg.tube_case(d, n_core=4, n_ring=2, n_axial=3, n_vel=8, vrange=5.0, delta=5.0,
                                                      accommodation=0.9, kinetic={"precision": "fp64"}),
