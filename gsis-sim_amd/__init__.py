@@ -14,6 +14,7 @@ Mirrors (reference tree):
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -106,7 +107,8 @@ class BackendOps(C.Structure):
 EXPORTS = [
     "gsk_last_error", "gsk_version", "gsk_setup_from_case", "gsk_setup_destroy", "gsk_setup_view_get",
     "gsk_setup_sizes", "gsk_setup_cell_new_to_old", "gsk_setup_storage_to_grid", "gsk_setup_cell_centers",
-    "gsk_setup_mass_flow_scale", "gsk_setup_write_pmesh", "gsk_backend_create", "gsk_backend_destroy",
+    "gsk_setup_mass_flow_scale", "gsk_setup_write_pmesh", "gsk_setup_describe", "gsk_setup_bface_geometry",
+    "gsk_backend_create", "gsk_backend_destroy",
     "gsk_backend_name", "gsk_backend_chunk_width", "gsk_backend_memory_bytes", "gsk_backend_set_equilibrium",
     "gsk_backend_set_distribution", "gsk_backend_get_distribution", "gsk_backend_compute_moments",
     "gsk_backend_scale_distribution", "gsk_backend_update_cell_params", "gsk_backend_set_inflow",
@@ -194,6 +196,24 @@ class Setup:
         out = C.c_double()
         _check(lib().gsk_setup_mass_flow_scale(self._h, C.byref(out)))
         return out.value
+
+    def describe(self) -> dict:
+        """Case name and files, sizes, Kn / delta, reference scales, [gsis] settings, [kinetic]
+        parameters and patches (gsk_setup_describe)."""
+        need = C.c_size_t()
+        _check(lib().gsk_setup_describe(self._h, None, C.c_size_t(0), C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _check(lib().gsk_setup_describe(self._h, buf, C.c_size_t(need.value), C.byref(need)))
+        return json.loads(buf.value.decode())
+
+    def bface_geometry(self):
+        """(patch index, area in mesh units^2, outward unit normal) per boundary face, original numbering."""
+        nb = self.n_bfaces
+        patch = np.empty(nb, np.int32)
+        area = np.empty(nb, np.float64)
+        normal = np.empty((nb, 3), np.float64)
+        _check(lib().gsk_setup_bface_geometry(self._h, _iptr(patch), _dptr(area), _dptr(normal)))
+        return patch, area, normal
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -2,6 +2,8 @@
 // turns them into a status code plus a thread-local message (the reference throws gsis::Error,
 // src/core/error.hpp:16-47; a C++ host rethrows the message).
 
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -50,6 +52,157 @@ int guarded(Fn&& fn) {
 
 void need(const void* p, const char* what) {
   if (p == nullptr) throw gsk::Error(std::string("null argument: ") + what);
+}
+
+// Minimal JSON object writer for the case description: keys in insertion order, doubles with 17
+// significant digits (they read back to the same value), strings escaped.
+class JsonOut {
+ public:
+  JsonOut& key(const char* k) {
+    sep();
+    str(k);
+    out_ += ':';
+    pending_value_ = true;
+    return *this;
+  }
+  JsonOut& val(double d) {
+    char b[40];
+    std::snprintf(b, sizeof(b), "%.17g", d);
+    std::string t = b;
+    if (t.find_first_of(".eEn") == std::string::npos) t += ".0";  // stays a real number when read back
+    return raw(t);
+  }
+  JsonOut& val(long long i) { return raw(std::to_string(i)); }
+  JsonOut& val(int i) { return raw(std::to_string(i)); }
+  JsonOut& val(bool v) { return raw(v ? "true" : "false"); }
+  JsonOut& val(const std::string& v) {
+    if (!pending_value_) sep();
+    pending_value_ = false;
+    str(v.c_str());
+    return *this;
+  }
+  JsonOut& open(char c) {
+    if (!pending_value_) sep();
+    pending_value_ = false;
+    out_ += c;
+    first_ = true;
+    return *this;
+  }
+  JsonOut& close(char c) {
+    out_ += c;
+    first_ = false;
+    return *this;
+  }
+  const std::string& text() const { return out_; }
+
+ private:
+  JsonOut& raw(const std::string& t) {
+    if (!pending_value_) sep();
+    pending_value_ = false;
+    out_ += t;
+    return *this;
+  }
+  void sep() {
+    if (!first_) out_ += ',';
+    first_ = false;
+  }
+  void str(const char* t) {
+    out_ += '"';
+    for (const char* c = t; *c; ++c) {
+      const unsigned char u = static_cast<unsigned char>(*c);
+      if (*c == '"' || *c == '\\') {
+        out_ += '\\';
+        out_ += *c;
+      } else if (u < 0x20) {
+        char b[8];
+        std::snprintf(b, sizeof(b), "\\u%04x", u);
+        out_ += b;
+      } else {
+        out_ += *c;
+      }
+    }
+    out_ += '"';
+  }
+  std::string out_;
+  bool first_ = true, pending_value_ = false;
+};
+
+const char* boundary_type_name(gsk::BoundaryType t) {
+  switch (t) {
+    case gsk::BoundaryType::MaxwellWall: return "maxwell_wall";
+    case gsk::BoundaryType::MassFlowInlet: return "mass_flow_inlet";
+    case gsk::BoundaryType::DiffuseWall: return "diffuse_wall";
+    case gsk::BoundaryType::PressureInlet: return "pressure_inlet";
+    case gsk::BoundaryType::PressureOutlet: return "pressure_outlet";
+    case gsk::BoundaryType::Vacuum: return "vacuum";
+    case gsk::BoundaryType::Symmetry: return "symmetry";
+    case gsk::BoundaryType::PeriodicSlab: return "periodic_slab";
+  }
+  return "unknown";
+}
+
+// What a cis-run driver writes into its result files besides the iteration records (the case,
+// sizes, scales, [kinetic] parameters and patches: src/app/cmd_cis_run.cpp:360-443), as JSON.
+std::string describe_json(const gsk::KineticSetup& k) {
+  const gsk::DerivedQuantities& d = k.derived;
+  const gsk::KineticParams& p = k.params;
+  long long cut = 0;  // internal faces between sweep blocks (block_partition_stats, src/mesh/ordering.cpp:265-286)
+  gsk::Index min_cells = k.n_cells(), max_cells = 0;
+  {
+    std::vector<int> block(static_cast<std::size_t>(k.n_cells()));
+    for (int b = 0; b < k.n_blocks(); ++b) {
+      const gsk::Index lo = k.block_offsets[static_cast<std::size_t>(b)], hi = k.block_offsets[static_cast<std::size_t>(b) + 1];
+      min_cells = std::min(min_cells, hi - lo);
+      max_cells = std::max(max_cells, hi - lo);
+      for (gsk::Index c = lo; c < hi; ++c) block[static_cast<std::size_t>(c)] = b;
+    }
+    for (gsk::Index f = 0; f < k.mesh.n_internal_faces(); ++f)
+      if (block[static_cast<std::size_t>(k.mesh.owner[static_cast<std::size_t>(f)])] !=
+          block[static_cast<std::size_t>(k.mesh.neighbour[static_cast<std::size_t>(f)])])
+        ++cut;
+  }
+  JsonOut j;
+  j.open('{');
+  j.key("case").val(k.case_name).key("mesh").val(k.mesh_file);
+  j.key("n_cells").val(static_cast<long long>(k.n_cells())).key("n_boundary_faces").val(static_cast<long long>(k.n_bfaces()));
+  j.key("n_velocity").val(static_cast<long long>(k.nv())).key("orbit_size").val(k.velocity.orbit_size);
+  j.key("Kn").val(d.Kn).key("delta").val(d.delta_from_Kn).key("has_p_in").val(d.has_p_in);
+  j.key("mass_flow_scale_kg_s").val(d.mass_flow_scale).key("length_scale").val(k.length_scale);
+  j.key("closed_domain").val(k.closed_domain);
+  j.key("tolerance").val(k.gsis_tolerance).key("max_iterations").val(p.max_iterations);
+  j.key("mass_balance_tolerance").val(k.mass_balance_tolerance).key("mass_balance_required").val(k.mass_balance_required);
+  j.key("sweep_blocks").val(static_cast<long long>(k.n_blocks())).key("block_cut_faces").val(cut);
+  j.key("block_min_cells").val(static_cast<long long>(min_cells)).key("block_max_cells").val(static_cast<long long>(max_cells));
+  j.key("reference").open('{');
+  j.key("length_m").val(d.ref_length).key("temperature_K").val(d.ref_temperature).key("pressure_Pa").val(d.ref_pressure);
+  j.key("density_kg_m3").val(d.ref_density).key("velocity_m_s").val(d.ref_velocity);
+  j.key("heat_flux_W_m2").val(d.ref_heat_flux).key("mass_flux_kg_m2_s").val(d.ref_mass_flux);
+  j.close('}');
+  j.key("params").open('{');
+  j.key("pseudo_cfl").val(p.pseudo_cfl).key("pseudo_time").val(p.pseudo_time).key("limiter").val(p.limiter);
+  j.key("limiter_k").val(p.limiter_k).key("limiter_eps").val(p.limiter_eps).key("lsq_weight_power").val(p.lsq_weight_power);
+  j.key("cell_ordering").val(p.cell_ordering).key("initial_pressure").val(p.initial_pressure);
+  j.key("initial_temperature").val(p.initial_temperature).key("convergence_u_floor").val(p.convergence_u_floor);
+  j.key("wall_relaxation").val(p.wall_relaxation).key("boundary_iterations").val(p.boundary_iterations);
+  j.key("interface_damping").val(p.interface_damping).key("positivity_limiter").val(p.positivity_limiter);
+  j.key("sweep_order").val(p.sweep_order).key("inner_sweeps").val(p.inner_sweeps);
+  j.key("inner_wall_update").val(p.inner_wall_update).key("block_passes").val(p.block_passes);
+  j.key("equilibrium").val(p.equilibrium).key("deq_fallback").val(p.deq_fallback);
+  j.key("pressure_boundary_reconstruction").val(p.pressure_boundary_reconstruction);
+  j.key("cold_bgk_temperature").val(p.cold_bgk_temperature).key("precision").val(p.precision);
+  j.key("backend").val(p.backend).key("moment_sums").val(p.moment_sums);
+  j.close('}');
+  j.key("patches").open('[');
+  for (const gsk::PatchCondition& pc : k.patches) {
+    j.open('{');
+    j.key("name").val(pc.name).key("type").val(std::string(boundary_type_name(pc.type)));
+    j.key("n_faces").val(static_cast<long long>(pc.n_faces)).key("accommodation").val(pc.accommodation);
+    j.key("p").val(pc.p).key("T").val(pc.T).key("target_mass_flow").val(pc.target_mass_flow);
+    j.close('}');
+  }
+  j.close(']');
+  j.close('}');
+  return j.text();
 }
 }  // namespace
 
@@ -137,6 +290,36 @@ int gsk_setup_write_pmesh(const gsk_setup* s, const char* path) {
     need(s, "setup");
     need(path, "path");
     gsk::write_pmesh(path, s->s.mesh);
+  });
+}
+
+int gsk_setup_describe(const gsk_setup* s, char* buf, size_t cap, size_t* needed) {
+  return guarded([&] {
+    need(s, "setup");
+    const std::string text = describe_json(s->s);
+    if (needed) *needed = text.size() + 1;
+    if (buf == nullptr || cap == 0) return;
+    GSK_CHECK(cap > text.size(), "description buffer too small");
+    std::memcpy(buf, text.c_str(), text.size() + 1);
+  });
+}
+
+int gsk_setup_bface_geometry(const gsk_setup* s, int32_t* patch, double* area, double* normal) {
+  return guarded([&] {
+    need(s, "setup");
+    const gsk::KineticSetup& k = s->s;
+    const gsk::Index n_int = k.mesh.n_internal_faces();
+    for (gsk::Index b = 0; b < k.n_bfaces(); ++b) {
+      const auto o = static_cast<std::size_t>(k.bface_new_to_old[static_cast<std::size_t>(b)]);
+      const auto f = static_cast<std::size_t>(n_int + b);
+      if (patch) patch[o] = k.bface_patch[static_cast<std::size_t>(b)];
+      if (area) area[o] = k.geometry.face_area[f];
+      if (normal) {
+        normal[3 * o + 0] = k.geometry.face_normal[f].x;
+        normal[3 * o + 1] = k.geometry.face_normal[f].y;
+        normal[3 * o + 2] = k.geometry.face_normal[f].z;
+      }
+    }
   });
 }
 

@@ -404,6 +404,14 @@ CaseConfig parse_case_toml(const std::string& text, const std::string& source) {
     const TomlTable& g = doc.tables.at("gsis");
     auto it = g.find("tolerance");
     if (it != g.end()) cfg.gsis_tolerance = it->second.number();
+    it = g.find("mass_balance_tolerance");
+    if (it != g.end()) cfg.gsis_mass_balance_tolerance = it->second.number();
+    require_positive(cfg.gsis_mass_balance_tolerance, "gsis.mass_balance_tolerance");
+    it = g.find("mass_balance_required");
+    if (it != g.end()) {
+      GSK_CHECK(it->second.kind == TomlValue::Bool, "gsis.mass_balance_required must be true or false");
+      cfg.gsis_mass_balance_required = it->second.b;
+    }
   }
   if (doc.tables.count("kinetic")) cfg.kinetic = doc.tables.at("kinetic");
   return cfg;
@@ -441,6 +449,7 @@ DerivedQuantities derive_quantities(const CaseConfig& c) {
   d.ref_density = d.ref_pressure / (d.R * d.ref_temperature);
   d.ref_velocity = std::sqrt(d.R * d.ref_temperature);
   d.ref_mass_flux = d.ref_density * d.ref_velocity;
+  d.ref_heat_flux = d.ref_pressure * d.ref_velocity;
   const double ref_viscosity = d.ref_pressure * d.ref_length / d.ref_velocity;
   d.mu_hat0 = d.mu0 / ref_viscosity;
   d.mass_flow_scale = kSqrtPi * c.L_ref * c.L_ref * d.p_in / d.most_probable_speed;

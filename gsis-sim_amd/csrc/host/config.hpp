@@ -68,6 +68,8 @@ struct CaseConfig {
   TomlTable kinetic;
   // [gsis] values used by CIS-only runs
   double gsis_tolerance = 1e-5;
+  double gsis_mass_balance_tolerance = 2e-3;  // [gsis].mass_balance_tolerance (cis-run's balance check)
+  bool gsis_mass_balance_required = false;    // [gsis].mass_balance_required
 
   double gas_constant() const { return kBoltzmann / molecular_mass; }
   double viscosity(double T) const { return mu_ref * std::pow(T / T_ref, omega); }
@@ -84,7 +86,7 @@ struct DerivedQuantities {
   bool has_p_in = false;
   // reference scales
   double ref_length = 0, ref_temperature = 0, ref_pressure = 0, ref_density = 0, ref_velocity = 0;
-  double ref_mass_flux = 0;
+  double ref_mass_flux = 0, ref_heat_flux = 0;
 };
 DerivedQuantities derive_quantities(const CaseConfig& c);
 

@@ -218,6 +218,11 @@ std::vector<std::string> check_case_against_mesh(const CaseConfig& cfg, const Po
 KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh, const KineticParams& params) {
   KineticSetup s;
   s.params = params;
+  s.case_name = config.name;
+  s.mesh_file = config.mesh_path();
+  s.gsis_tolerance = config.gsis_tolerance;
+  s.mass_balance_tolerance = config.gsis_mass_balance_tolerance;
+  s.mass_balance_required = config.gsis_mass_balance_required;
   s.limiter_mode = params.limiter == "none" ? 0 : (params.limiter_eps == "relative" ? 2 : 1);
   s.derived = derive_quantities(config);
   s.length_scale = config.L_ref;

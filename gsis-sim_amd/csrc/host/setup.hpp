@@ -110,6 +110,10 @@ struct PatchCondition {
 
 struct KineticSetup {
   KineticParams params;
+  // case identity and the [gsis] convergence settings a cis-run driver reads (result files)
+  std::string case_name, mesh_file;
+  double gsis_tolerance = 1e-5, mass_balance_tolerance = 2e-3;
+  bool mass_balance_required = false;
   int limiter_mode = 1;
   DerivedQuantities derived;
   double length_scale = 1.0;

@@ -152,6 +152,16 @@ int gsk_setup_storage_to_grid(const gsk_setup* s, int32_t* out);          /* nv 
 int gsk_setup_cell_centers(const gsk_setup* s, double* out);              /* 3 n_cells, internal numbering, m */
 int gsk_setup_mass_flow_scale(const gsk_setup* s, double* out);           /* nondimensional -> Mdot denominator */
 int gsk_setup_write_pmesh(const gsk_setup* s, const char* path);          /* renumbered mesh (diagnostics) */
+/* What a cis-run driver records besides the iteration data (src/app/cmd_cis_run.cpp:360-443,
+ * src/kinetic/kinetic_output.cpp:198-389): a JSON object with the case name and mesh file, sizes,
+ * Kn and delta, the reference scales, the [gsis] tolerance and mass-balance settings, the sweep-block
+ * statistics, the [kinetic] parameters and the patches (name, type, faces, accommodation, p, T,
+ * target mass flow) in patch order. Written NUL-terminated into buf (cap bytes); *needed = its
+ * length + 1 (call with buf = NULL to size it). */
+int gsk_setup_describe(const gsk_setup* s, char* buf, size_t cap, size_t* needed);
+/* Per boundary face in the ORIGINAL boundary-face numbering: patch index, area (mesh units^2) and
+ * outward unit normal (3 per face); any output may be NULL. */
+int gsk_setup_bface_geometry(const gsk_setup* s, int32_t* patch, double* area, double* normal);
 
 /* ---------------------------------------------------------------- backend --------------- */
 typedef struct gsk_backend gsk_backend;
