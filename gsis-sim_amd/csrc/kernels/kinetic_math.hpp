@@ -54,14 +54,19 @@ GSK_HD float nmax(float a, float b) { return ::fmaxf(a, b); }
 GSK_HD double nmin(double a, double b) { return ::fmin(a, b); }
 GSK_HD float nmin(float a, float b) { return ::fminf(a, b); }
 
-// Dot products and multiply-adds in the form each precision computes them: FP64 keeps the
-// reference's separately rounded products and left-to-right sums (the drop-in's parity at the ulp
-// level); FP32 fuses them (one v_fma_f32 per term), which the FP32 form's tolerance admits and which
-// cuts the vector instructions of these expressions by up to half.
-GSK_HD double dot3(double x, double y, double z, double a, double b, double c) { return x * a + y * b + z * c; }
-GSK_HD float dot3(float x, float y, float z, float a, float b, float c) { return ::fmaf(z, c, ::fmaf(y, b, x * a)); }
-GSK_HD double madd(double acc, double a, double b) { return acc + a * b; }
-GSK_HD float madd(float acc, float a, float b) { return ::fmaf(a, b, acc); }
+// Dot products and multiply-adds with separately rounded products and left-to-right sums, the
+// reference's form, in both precisions (the files build with -ffp-contract=off). FP64 needs it for
+// ulp-level parity. In FP32, fused forms (one v_fma_f32 per term) saved ~1% of a step but raised the
+// iteration-to-iteration jitter of a converged FP32 solution about six-fold (the convergence measure
+// of eq. (59) then stalls near 2e-6 on the reference's slab cases instead of 3e-7 to 5e-7).
+template <class T>
+GSK_HD T dot3(T x, T y, T z, T a, T b, T c) {
+  return x * a + y * b + z * c;
+}
+template <class T>
+GSK_HD T madd(T acc, T a, T b) {
+  return acc + a * b;
+}
 
 // f_eq = A exp(-B |xi - u|^2)
 template <class F>

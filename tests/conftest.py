@@ -118,3 +118,47 @@ def rel_err(a, b, floor=1e-300):
     b = np.asarray(b, np.float64)
     scale = np.max(np.abs(b)) if b.size else 0.0
     return float(np.max(np.abs(a - b)) / max(scale, floor)) if a.size else 0.0
+
+
+# The reference's slab verification cases (cases/verification/kinetic/make_cases.py): walls at
+# T_L/T0, T_R/T0 moving with u_y = -/+ u sqrt(R T0), every side periodic (the `_per` runs), 30 x 24 x 24
+# velocities on [-10, 10] x [-8, 8]^2, Kn from eq. (8), one sweep block.
+SLAB_KINDS = {"heat_lin": (0.95, 1.05, 0.0), "heat_nl": (0.5, 1.5, 0.0), "couette_lin": (1.0, 1.0, 0.1),
+              "couette_nl": (1.0, 1.0, 1.0)}
+
+
+def slab_verification_case(mg, directory, kind, kn, nx, precision="fp64"):
+    c0 = (1.380649e-23 / 6.65e-27 * T0) ** 0.5
+    tl, tr, u = SLAB_KINDS[kind]
+    mesh_file = f"slab_nx{nx}.pmesh"
+    mg.write_pmesh(os.path.join(directory, mesh_file), mg.slab_mesh(nx), {"generator": "gsis-sim_amd.meshgen.slab_mesh"})
+    walls = []
+    for patch, t, sgn in (("wall_left", tl, -1.0), ("wall_right", tr, 1.0)):
+        w = dict(patch=patch, type="diffuse_wall", temperature=t * T0)
+        if u != 0.0:
+            w["velocity"] = [0.0, sgn * u * c0, 0.0]
+        walls.append(w)
+    bnd = walls + [dict(patch="sym_y", type="periodic_slab"), dict(patch="sym_z", type="periodic_slab")]
+    name = f"slab_{kind}_Kn{kn:g}_nx{nx}_{precision}".replace(".", "p")
+    path = os.path.join(directory, name + ".toml")
+    mg.write_case(path, mesh_file, name, bnd, [30, 24, 24], [(-10.0, 10.0), (-8.0, 8.0), (-8.0, 8.0)],
+                  dict(L_ref=1.0e-3, T0=T0, Kn=float(kn)), dict(sweep_blocks=1, precision=precision))
+    return path
+
+
+def slab_reference_rows():
+    """The reference's converged periodic slab runs (tests/golden/slab_reference_results.csv)."""
+    import csv
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "slab_reference_results.csv")) as f:
+        return {r["case"]: r for r in csv.DictReader(f)}
+
+
+def slab_mean_flux(solver, setup, kind):
+    """Volume-weighted mean over the cells of P_xy (Couette) or q_x (heat transfer): the value the
+    reference's compare_ref1d.py takes from cells.csv."""
+    m = solver.moments()
+    vol = np.empty(setup.n_cells)
+    vol[setup.cell_new_to_old()] = np.ctypeslib.as_array(setup.view().volume, shape=(setup.n_cells,))
+    w = vol / vol.sum()
+    col = 9 if kind.startswith("couette") else 12  # sigma_xy, q_x in the 15-moment row
+    return float(np.sum(w * m[:, col]))
