@@ -121,13 +121,14 @@ def rel_err(a, b, floor=1e-300):
 
 
 # The reference's slab verification cases (cases/verification/kinetic/make_cases.py): walls at
-# T_L/T0, T_R/T0 moving with u_y = -/+ u sqrt(R T0), every side periodic (the `_per` runs), 30 x 24 x 24
-# velocities on [-10, 10] x [-8, 8]^2, Kn from eq. (8), one sweep block.
+# T_L/T0, T_R/T0 moving with u_y = -/+ u sqrt(R T0); sides all periodic (the `_per` runs) or, by
+# default, symmetry planes (heat transfer) / sym_y periodic and sym_z a symmetry plane (Couette flow);
+# 30 x 24 x 24 velocities on [-10, 10] x [-8, 8]^2, Kn from eq. (8), one sweep block.
 SLAB_KINDS = {"heat_lin": (0.95, 1.05, 0.0), "heat_nl": (0.5, 1.5, 0.0), "couette_lin": (1.0, 1.0, 0.1),
               "couette_nl": (1.0, 1.0, 1.0)}
 
 
-def slab_verification_case(mg, directory, kind, kn, nx, precision="fp64"):
+def slab_verification_case(mg, directory, kind, kn, nx, precision="fp64", sides="periodic"):
     c0 = (1.380649e-23 / 6.65e-27 * T0) ** 0.5
     tl, tr, u = SLAB_KINDS[kind]
     mesh_file = f"slab_nx{nx}.pmesh"
@@ -138,8 +139,12 @@ def slab_verification_case(mg, directory, kind, kn, nx, precision="fp64"):
         if u != 0.0:
             w["velocity"] = [0.0, sgn * u * c0, 0.0]
         walls.append(w)
-    bnd = walls + [dict(patch="sym_y", type="periodic_slab"), dict(patch="sym_z", type="periodic_slab")]
-    name = f"slab_{kind}_Kn{kn:g}_nx{nx}_{precision}".replace(".", "p")
+    if sides == "periodic":
+        sy, sz = "periodic_slab", "periodic_slab"
+    else:
+        sy, sz = ("periodic_slab" if kind.startswith("couette") else "symmetry"), "symmetry"
+    bnd = walls + [dict(patch="sym_y", type=sy), dict(patch="sym_z", type=sz)]
+    name = f"slab_{kind}_Kn{kn:g}_nx{nx}_{sides}_{precision}".replace(".", "p")
     path = os.path.join(directory, name + ".toml")
     mg.write_case(path, mesh_file, name, bnd, [30, 24, 24], [(-10.0, 10.0), (-8.0, 8.0), (-8.0, 8.0)],
                   dict(L_ref=1.0e-3, T0=T0, Kn=float(kn)), dict(sweep_blocks=1, precision=precision))
