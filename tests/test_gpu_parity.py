@@ -3,8 +3,10 @@
 FP64 (the drop-in form): every backend method within 1e-12 relative (the only differences are the
 order of FP64 reductions over velocities and last-ulp differences of exp), and whole CIS iterations
 within 1e-10 relative (the bar the reference's own GPU-vs-CPU test uses with default settings,
-tests/kinetic/test_kinetic_gpu.cpp:133-158). FP32 f: within 2e-5 relative of the FP32 oracle (the
-reference's FP32 tolerance). Also: determinism and velocity-chunk independence, bitwise."""
+tests/kinetic/test_kinetic_gpu.cpp:133-158). FP32 f: every method within 2e-6 and five CIS
+iterations within 1e-6 of the FP32 oracle (measured 8.5e-7 and 9e-8; both compute in the reference's
+operation order, so only reduction order and last-ulp exp / division differ; the reference's own
+FP32 tolerance is 2e-5). Also: determinism and velocity-chunk independence, bitwise."""
 
 from __future__ import annotations
 
@@ -65,10 +67,10 @@ METHOD_CASES = {
                                                          "gsis_negative_check": "limit", "sweep_order": "dir13",
                                                          "positivity_limiter": True, "block_passes": 3,
                                                          "velocity_chunk": 64}), 1e-12),
-    "box_mixed_fp32": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}), 2e-5),
+    "box_mixed_fp32": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}), 2e-6),
     # tanh-stretched velocity grid (an extension; the oracle carries the same per-velocity weights)
     "box_mixed_fp64_stretched": (dict(kinetic={"precision": "fp64"}, stretch=1.5), 1e-12),
-    "box_mixed_fp32_stretched": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}, stretch=1.5), 2e-5),
+    "box_mixed_fp32_stretched": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}, stretch=1.5), 2e-6),
 }
 
 
@@ -201,7 +203,9 @@ def test_fp32_form_tracks_fp32_oracle(gsk, meshgen, oracle_mod, tmp_path):
     cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
     cpu = _run_solver(gsk, setup, cb.ops, 5)
     gpu = _run_solver(gsk, setup, None, 5)
-    assert rel_err(gpu["moments"], cpu["moments"]) <= 2e-5
+    assert rel_err(gpu["moments"], cpu["moments"]) <= 1e-6
+    assert rel_err(gpu["f"], cpu["f"]) <= 2e-6
+    assert rel_err(gpu["hist"][:, 0], cpu["hist"][:, 0]) <= 1e-6
 
 
 def test_deterministic_and_chunk_independent(gsk, meshgen, tmp_path):
