@@ -172,10 +172,11 @@ def slab_mesh(nx: int = 200, length: float = 1.0e-3, thickness: float | None = N
 
 
 def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: float = 0.0,
-             seed: int = 0) -> dict:
+             seed: int = 0, names=None) -> dict:
     """Structured box of hexahedra, optionally with interior nodes moved by up to `distort` x the
     local spacing (seeded numpy generator; boundary nodes stay on their planes so every boundary
-    face keeps an axis-aligned normal). Patches xmin, xmax, ymin, ymax, zmin, zmax."""
+    face keeps an axis-aligned normal). Patches xmin, xmax, ymin, ymax, zmin, zmax, or the six
+    `names` in that order (sides with the same name form one patch)."""
     n = (nx, ny, nz)
     axes = [np.linspace(0.0, size[a], n[a] + 1) for a in range(3)]
     I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
@@ -194,16 +195,20 @@ def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: 
             for k in range(nz):
                 cells.append([nid(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) for c in range(8)])
 
+    side_names = list(names) if names is not None else ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
+    assert len(side_names) == 6
+    unique = list(dict.fromkeys(side_names))
+    side_patch = [unique.index(n) for n in side_names]
+
     def patch_of(fc, nrm):
         p = np.full(len(fc), -1)
         for a in range(3):
-            p[fc[:, a] < 1e-9 * size[a]] = 2 * a
-            p[fc[:, a] > size[a] * (1 - 1e-9)] = 2 * a + 1
+            p[fc[:, a] < 1e-9 * size[a]] = side_patch[2 * a]
+            p[fc[:, a] > size[a] * (1 - 1e-9)] = side_patch[2 * a + 1]
         assert (p >= 0).all()
         return p
 
-    names = ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
-    return hex_to_poly(pts, np.array(cells), patch_of, names, ["patch"] * 6)
+    return hex_to_poly(pts, np.array(cells), patch_of, unique, ["patch"] * len(unique))
 
 
 def tube_mesh(n_core: int = 16, n_ring: int = 10, n_axial: int = 45, radius: float = 1.0e-3,

@@ -167,3 +167,35 @@ def slab_mean_flux(solver, setup, kind):
     w = vol / vol.sum()
     col = 9 if kind.startswith("couette") else 12  # sigma_xy, q_x in the 15-moment row
     return float(np.sum(w * m[:, col]))
+
+
+def kinetic_case(mg, directory, mesh, bcs, Kn=0.5, n_vel=10, vmax=5.0, kinetic=None, name="kcase"):
+    """The reference's make_kinetic_case (tests/kinetic/kinetic_test_cases.cpp:9-60): L_ref = 1 m,
+    T0 = 273.15 K, helium, the given Kn, n_vel^3 velocities on [-vmax, vmax]^3; boundary values in
+    reference units (T / T0, p / p_ref, u / sqrt(R T0)), converted with the case's own scales."""
+    mg.write_pmesh(os.path.join(directory, f"{name}.pmesh"), mesh, {"generator": "tests/conftest.py"})
+    path = os.path.join(directory, f"{name}.toml")
+
+    def write(p_ref, v_ref):
+        out = []
+        for b in bcs:
+            d = dict(patch=b["patch"], type=b["type"])
+            if b["type"] == "diffuse_wall":
+                d["temperature"] = b.get("T", 1.0) * T0
+                u = b.get("u", (0.0, 0.0, 0.0))
+                if any(x != 0.0 for x in u):
+                    d["velocity"] = [x * v_ref for x in u]
+            elif b["type"] == "pressure_inlet":
+                d["temperature"] = b.get("T", 1.0) * T0
+                d["pressure"] = b.get("p", 1.0) * p_ref
+            elif b["type"] == "pressure_outlet":
+                d["pressure"] = b.get("p", 1.0) * p_ref
+            out.append(d)
+        mg.write_case(path, f"{name}.pmesh", name, out, [n_vel] * 3, [(-vmax, vmax)] * 3,
+                      dict(L_ref=1.0, T0=T0, Kn=float(Kn)), kinetic)
+
+    write(1.0, 1.0)  # probe for the reference scales
+    import gsis_sim_amd as g  # registered by load_package
+    ref = g.Setup(path).describe()["reference"]
+    write(ref["pressure_Pa"], ref["velocity_m_s"])
+    return path

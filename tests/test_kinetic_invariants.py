@@ -1,0 +1,110 @@
+"""Solution-level invariants from the reference's own kinetic tests, on the oracle (CPU) and on the
+HIP backend (GPU suite), in FP64 and in FP32 (the reference's default precision, in which it runs
+these tests):
+
+- a heat-conducting slab gives the same solution with symmetry-plane and with periodic sides
+  (tests/kinetic/test_kinetic_cpu.cpp:142-175);
+- Couette flow between walls moving at -/+0.3 with periodic sides is antisymmetric in u_y and
+  symmetric in T, with P_xy < 0 everywhere and wall slip (test_kinetic_cpu.cpp:177-199);
+- the converged solution does not depend on the sweep order, the inner sweeps, the wall update or
+  the block passes (tests/kinetic/test_kinetic_phase5.cpp:200-239).
+The meshes are this build's box generator (same sizes and patches; the reference's distorted-box
+node jitter uses its own random numbers, so the third mesh differs in its interior nodes)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import kinetic_case
+
+SLAB = ["wall_left", "wall_right", "sym_y", "sym_y", "sym_z", "sym_z"]
+BACKENDS = ["oracle", pytest.param("hip", marks=pytest.mark.gpu)]
+PRECISIONS = ["fp64", "fp32"]
+
+
+def solve(gsk, oracle_mod, path, backend, iterations):
+    setup = gsk.Setup(path)
+    keep = None
+    if backend == "oracle":
+        keep = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps)
+        sv = gsk.Solver(setup, keep.ops)
+    else:
+        sv = gsk.Solver(setup)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    eps = 1.0
+    for _ in range(iterations):
+        eps = sv.prepare()["epsilon"]
+        sv.sweep()
+    m = sv.moments()
+    sv.close()
+    return m, eps
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_symmetry_and_periodic_slab_agree(gsk, meshgen, oracle_mod, tmp_path, backend, precision):
+    print(f"[{backend}] symmetry vs periodic slab {precision}", flush=True)
+    mesh = meshgen.box_mesh(8, 1, 1, size=(1.0, 0.125, 0.125), names=SLAB)
+    out = {}
+    for side in ("symmetry", "periodic_slab"):
+        path = kinetic_case(meshgen, str(tmp_path), mesh,
+                            [dict(patch="wall_left", type="diffuse_wall", T=0.9),
+                             dict(patch="wall_right", type="diffuse_wall", T=1.1),
+                             dict(patch="sym_y", type=side), dict(patch="sym_z", type=side)],
+                            Kn=0.2, kinetic=dict(sweep_blocks=1, precision=precision), name=f"slab_{side}")
+        out[side], _ = solve(gsk, oracle_mod, path, backend, 150)
+    a, b = out["symmetry"], out["periodic_slab"]
+    assert np.allclose(a[:, 4], b[:, 4], rtol=2e-5, atol=0)  # T
+    assert np.allclose(a[:, 0], b[:, 0], rtol=2e-5, atol=0)  # rho
+    assert np.allclose(a[:, 12], b[:, 12], rtol=2e-3, atol=0)  # q_x
+    assert np.max(np.abs(a[:, 2])) < 1e-6 and np.max(np.abs(b[:, 3])) < 1e-6
+    q = a[:, 12]
+    assert q.max() < 0.0  # heat flows from the hot (right) wall to the cold one
+    assert q.max() - q.min() < 0.1 * abs(q.min())
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_couette_with_periodic_sides_is_antisymmetric(gsk, meshgen, oracle_mod, tmp_path, backend, precision):
+    print(f"[{backend}] couette antisymmetry {precision}", flush=True)
+    mesh = meshgen.box_mesh(8, 1, 1, size=(1.0, 0.125, 0.125), names=SLAB)
+    path = kinetic_case(meshgen, str(tmp_path), mesh,
+                        [dict(patch="wall_left", type="diffuse_wall", u=(0.0, -0.3, 0.0)),
+                         dict(patch="wall_right", type="diffuse_wall", u=(0.0, 0.3, 0.0)),
+                         dict(patch="sym_y", type="periodic_slab"), dict(patch="sym_z", type="symmetry")],
+                        Kn=0.2, kinetic=dict(sweep_blocks=1, precision=precision), name="couette")
+    m, _ = solve(gsk, oracle_mod, path, backend, 150)
+    uy, T = m[:, 2], m[:, 4]
+    assert np.max(np.abs(uy + uy[::-1])) <= 2e-5
+    assert np.max(np.abs(T - T[::-1])) <= 2e-5
+    assert np.all(m[:, 9] < 0.0)  # P_xy < 0: the left wall moves to -y, the right one to +y
+    assert 0.0 < uy[-1] < 0.3  # slip at the wall
+
+
+ORDERINGS = [("block", 1, False, 1), ("octant", 1, False, 1), ("dir13", 1, False, 1), ("block", 3, True, 1),
+             ("octant", 2, False, 1), ("octant", 1, False, 3), None]  # None: the defaults
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_sweep_orderings_converge_to_the_same_solution(gsk, meshgen, oracle_mod, tmp_path, backend, precision):
+    print(f"[{backend}] sweep orderings {precision}", flush=True)
+    mesh = meshgen.box_mesh(6, 3, 3, size=(1.2, 0.6, 0.6), distort=0.15, seed=4,
+                            names=["inlet", "outlet", "wall", "wall", "wall", "wall"])
+    bcs = [dict(patch="inlet", type="pressure_inlet"), dict(patch="outlet", type="vacuum"),
+           dict(patch="wall", type="diffuse_wall")]
+    sols = []
+    for i, spec in enumerate(ORDERINGS):
+        kin = dict(sweep_blocks=2, precision=precision)
+        if spec is not None:
+            kin.update(sweep_order=spec[0], inner_sweeps=spec[1], inner_wall_update=spec[2], block_passes=spec[3])
+        path = kinetic_case(meshgen, str(tmp_path), mesh, bcs, Kn=0.3, kinetic=kin, name=f"order{i}")
+        m, eps = solve(gsk, oracle_mod, path, backend, 400 if i == 0 else 300)
+        assert eps < 1e-6, (spec, eps)
+        sols.append(m)
+    a = sols[0]
+    for spec, b in zip(ORDERINGS[1:], sols[1:]):
+        assert np.allclose(b[:, 0], a[:, 0], rtol=2e-5, atol=0), spec
+        assert np.allclose(b[:, 4], a[:, 4], rtol=2e-5, atol=0), spec
+        assert np.max(np.abs(b[:, 1] - a[:, 1])) <= 2e-5, spec
