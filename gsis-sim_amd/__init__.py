@@ -118,7 +118,7 @@ EXPORTS = [
     "gsk_solver_initialize_equilibrium", "gsk_solver_prepare", "gsk_solver_update_mass_flow_pressure",
     "gsk_solver_sweep", "gsk_solver_patch_mass_flow", "gsk_solver_moments", "gsk_solver_boundary_fluxes",
     "gsk_solver_apply_gsis_correction", "gsk_solver_distribution", "gsk_solver_set_distribution",
-    "gsk_solver_residual_per_cell", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
+    "gsk_solver_residual_per_cell", "gsk_solver_section_mass_flow", "gsk_solver_section_cuts", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
     "gsk_solver_set_distribution_device", "gsk_solver_residual_device", "gsk_solver_precondition_device",
     "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
     "gsk_solver_set_boundary_iterations",
@@ -415,6 +415,24 @@ class Solver:
         out = np.empty(self.setup.n_cells, np.float64)
         _check(lib().gsk_solver_residual_per_cell(self._h, _dptr(out), C.c_size_t(out.size)))
         return out
+
+    def section_mass_flow(self, positions, axis: int = 0) -> np.ndarray:
+        """Nondimensional mass flow along +axis through the sections at `positions` (L_ref units)."""
+        pos = np.ascontiguousarray(positions, np.float64)
+        out = np.empty(pos.size, np.float64)
+        _check(lib().gsk_solver_section_mass_flow(self._h, C.c_int32(axis), _dptr(pos), C.c_size_t(pos.size),
+                                                  _dptr(out)))
+        return out
+
+    def section_cuts(self, positions, axis: int = 0) -> dict:
+        """Per section: face count, area, projected area (L_ref^2 units), area-weighted face position."""
+        pos = np.ascontiguousarray(positions, np.float64)
+        n = np.empty(pos.size, np.int64)
+        area, proj, fpos = (np.empty(pos.size, np.float64) for _ in range(3))
+        _check(lib().gsk_solver_section_cuts(self._h, C.c_int32(axis), _dptr(pos), C.c_size_t(pos.size),
+                                             n.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(area), _dptr(proj),
+                                             _dptr(fpos)))
+        return dict(n_faces=n, area=area, projected_area=proj, face_position=fpos)
 
     # ---- device-resident operators (device pointers, n = cells x velocities values of element_size) ----
     def element_size(self) -> int:

@@ -12,9 +12,10 @@ scientific, whichever is shorter), std::to_chars plus ".0" for integral reals in
 (src/io/csv_writer.cpp:22-29, src/core/json.cpp:461-472), and the JSON layout of json::dump (two-space
 indent, arrays of numbers on one line).
 
-Not written: fields.vtu / boundary.vtu (visualisation), the section mass flows of
-[output].mass_flow_sections, and the STOP-file interrupt. cells.csv is written when asked
-(`cells_csv=True`, the reference's --cells-csv).
+The section mass flows of [output].mass_flow_sections (src/mesh/section_cut.hpp) go into both files
+as the reference writes them (history columns M_dot_sec<k>, summary mass_flow_sections and the
+balance record's section spread). Not written: fields.vtu / boundary.vtu (visualisation) and the
+STOP-file interrupt. cells.csv is written when asked (`cells_csv=True`, the reference's --cells-csv).
 
 Run as a script: python gsis-sim_amd/cis_run.py case.toml [--out DIR] [--max-iter N] [--min-iter N]
 [--tol X] [--precision fp32|fp64] [--set key=value]... [--print-every N] [--cells-csv]
@@ -205,9 +206,12 @@ def flow_spread(flows) -> float:
 TIMING_KEYS = ("moments", "boundary", "gradient", "residual", "sweep_forward", "sweep_backward", "other")
 
 
+def mass_flow_display_prefix(desc: dict) -> str:
+    return "M_dot_" if desc["has_p_in"] else "mdot_nd_"
+
+
 class HistoryWriter:
-    """history.csv with KineticHistoryWriter's columns (src/kinetic/kinetic_output.cpp:323-389);
-    no section columns (section cuts are not computed here)."""
+    """history.csv with KineticHistoryWriter's columns (src/kinetic/kinetic_output.cpp:323-389)."""
 
     def __init__(self, path: str, desc: dict, gpu_memory_bytes: int):
         self.desc = desc
@@ -220,7 +224,9 @@ class HistoryWriter:
         header += ["qx_mean", "qy_mean", "qz_mean", "sxx_mean", "syy_mean", "szz_mean", "sxy_mean", "syz_mean",
                    "sxz_mean"]
         header += ["time_iter_s", "time_total_s"] + [f"t_{k}_s" for k in TIMING_KEYS]
-        header += ["mass_imbalance", "gpu_memory_bytes"]
+        self.n_sections = len(desc["sections"]["positions"])
+        header += ["mass_imbalance"] + [f"{mass_flow_display_prefix(desc)}sec{k}" for k in range(self.n_sections)]
+        header += ["gpu_memory_bytes"]
         header += [f"p_b_{pc['name']}_Pa" for pc in desc["patches"] if pc["type"] == "mass_flow_inlet"]
         self.header = header
         self._f = open(path, "w", newline="")
@@ -230,7 +236,7 @@ class HistoryWriter:
         self._f.write(",".join(_csv_field(v) for v in values) + "\n")
         self._f.flush()
 
-    def write(self, info: dict, flows, elapsed_total_s: float) -> None:
+    def write(self, info: dict, flows, elapsed_total_s: float, sections=()) -> None:
         nan = math.nan
         eps = info["has_epsilon"]
         if info["has_residual"] and self.first_residual == 0.0:
@@ -247,6 +253,7 @@ class HistoryWriter:
         row += [info["seconds"], elapsed_total_s] + [info["timings"][k] for k in TIMING_KEYS]
         mb = mass_balance(flows)
         row.append(mb["imbalance"] if mb["valid"] else nan)
+        row += [mass_flow_display(self.desc, sections[k]) if k < len(sections) else nan for k in range(self.n_sections)]
         row.append(self.gpu_memory_bytes)
         row += [pc["p"] * self.desc["reference"]["pressure_Pa"] for pc in self.desc["patches"]
                 if pc["type"] == "mass_flow_inlet"]
@@ -335,16 +342,19 @@ def tube_mass_flow(desc: dict, ps: dict):
     return mf
 
 
-def balance_record(desc: dict, patch_flows) -> dict:
-    """The kinetic entry of the mass-balance report (src/app/mass_balance_report.cpp:16-73), no
-    sections: inflow / outflow totals, imbalance, and a warning above the tolerance."""
+def balance_record(desc: dict, patch_flows, sections=()) -> dict:
+    """The kinetic entry of the mass-balance report (src/app/mass_balance_report.cpp:16-73):
+    inflow / outflow totals, imbalance, the spread of {inflow, outflow, |sections|}, and warnings
+    above the tolerance."""
     b = mass_balance(patch_flows)
-    spread = flow_spread([b["inflow"], b["outflow"]]) if b["valid"] else math.nan
+    spread = flow_spread([b["inflow"], b["outflow"]] + [abs(x) for x in sections]) if b["valid"] else math.nan
     tol = desc["mass_balance_tolerance"]
     warnings = []
     if b["valid"] and abs(b["imbalance"]) > tol:
         warnings.append(f"kinetic (last f): inlet / outlet mass-flow imbalance {b['imbalance']:+.3e} "
                         f"(tolerance {tol:.1e})")
+    if len(sections) and math.isfinite(spread) and spread > tol:
+        warnings.append(f"kinetic (last f): section / inlet / outlet mass-flow spread {spread:.3e} (tolerance {tol:.1e})")
     e = {"inflow_nd": b["inflow"], "outflow_nd": b["outflow"]}
     if desc["has_p_in"]:
         e["M_dot_inflow"] = mass_flow_display(desc, b["inflow"])
@@ -356,6 +366,24 @@ def balance_record(desc: dict, patch_flows) -> dict:
             "definition": "imbalance = (outflow - inflow) / max(inflow, outflow) from the outward boundary mass "
                           "flows; section_spread = (max - min) / mean of {inflow, outflow, |sections|}",
             "kinetic": e, "within_tolerance": not warnings, "warnings": warnings}
+
+
+def sections_record(desc: dict, cuts: dict, flows) -> dict:
+    """mass_flow_sections (src/app/mass_balance_report.cpp:75-100) for the kinetic f."""
+    out = []
+    for k, x in enumerate(desc["sections"]["positions"]):
+        e = {"index": k, "position": float(x), "face_position": float(cuts["face_position"][k]),
+             "n_faces": int(cuts["n_faces"][k]), "area_nd": float(cuts["area"][k]),
+             "projected_area_nd": float(cuts["projected_area"][k]),
+             "mdot_kinetic_nd": float(flows[k]) if math.isfinite(flows[k]) else None}
+        if desc["has_p_in"]:
+            m = mass_flow_display(desc, float(flows[k]))
+            e["M_dot_kinetic"] = m if math.isfinite(m) else None
+        out.append(e)
+    return {"axis": desc["sections"]["axis"],
+            "definition": "sum of the discrete face mass fluxes over the internal faces between cells whose centres lie "
+                          "before and after the position (positive along the axis); kinetic = the last f",
+            "sections": out}
 
 
 def write_cells_csv(path: str, setup: Setup, solver: Solver, desc: dict) -> None:
@@ -414,6 +442,12 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     memory_bytes = backend.memory_bytes()
     history = HistoryWriter(os.path.join(out_dir, "history.csv"), desc,
                             memory_bytes if params["backend"] == "cuda" else 0)
+    sec_pos = list(desc["sections"]["positions"])
+    sec_axis = "xyz".index(desc["sections"]["axis"])
+
+    def section_flows():
+        return list(solver.section_mass_flow(sec_pos, sec_axis)) if sec_pos else []
+
     converged = False
     timing_sum = dict.fromkeys(TIMING_KEYS, 0.0)
     timed, iter_time_sum = 0, 0.0
@@ -422,20 +456,21 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
         while True:
             prep = solver.prepare()
             flows = solver.patch_mass_flow()
+            sec = section_flows()
             mb = mass_balance(flows)
             balance_ok = (not desc["mass_balance_required"] or not mb["valid"]
                           or abs(mb["imbalance"]) < desc["mass_balance_tolerance"])
             if (prep["iteration"] >= min_iter and prep["has_epsilon"] and prep["epsilon"] < tol and balance_ok
                     and mass_flow_targets_met(desc, flows)):
                 converged, last = True, prep
-                history.write(prep, flows, time.perf_counter() - t_total)
+                history.write(prep, flows, time.perf_counter() - t_total, sec)
                 break
             if prep["iteration"] >= max_iter:
                 last = prep
-                history.write(prep, flows, time.perf_counter() - t_total)
+                history.write(prep, flows, time.perf_counter() - t_total, sec)
                 break
             info = solver.sweep()
-            history.write(info, flows, time.perf_counter() - t_total)
+            history.write(info, flows, time.perf_counter() - t_total, sec)
             if info["iteration"] >= 1:  # the first iteration, right after initialisation, is not timed
                 for k in TIMING_KEYS:
                     timing_sum[k] += info["timings"][k]
@@ -449,6 +484,7 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     wall_s = time.perf_counter() - t_total
 
     ps = summarize_patches(desc, setup.bface_geometry(), solver.boundary_fluxes())
+    sec_final = section_flows()
     if cells_csv:
         write_cells_csv(os.path.join(out_dir, "cells.csv"), setup, solver, desc)
     scale = 1.0 / timed if timed > 0 else 0.0
@@ -482,7 +518,9 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     mf = tube_mass_flow(desc, ps)
     if mf is not None:
         root["mass_flow"] = mf
-    root["mass_balance"] = balance_record(desc, ps["mdot"])
+    root["mass_balance"] = balance_record(desc, ps["mdot"], sec_final)
+    if sec_pos:
+        root["mass_flow_sections"] = sections_record(desc, solver.section_cuts(sec_pos, sec_axis), sec_final)
     root["residual"] = last["residual"] if math.isfinite(last["residual"]) else 0.0
     with open(os.path.join(out_dir, "summary.json"), "w") as f:
         f.write(dump_json(root) + "\n")

@@ -173,6 +173,10 @@ std::string describe_json(const gsk::KineticSetup& k) {
   j.key("mass_balance_tolerance").val(k.mass_balance_tolerance).key("mass_balance_required").val(k.mass_balance_required);
   j.key("sweep_blocks").val(static_cast<long long>(k.n_blocks())).key("block_cut_faces").val(cut);
   j.key("block_min_cells").val(static_cast<long long>(min_cells)).key("block_max_cells").val(static_cast<long long>(max_cells));
+  j.key("sections").open('{').key("axis").val(std::string(k.section_axis == 0 ? "x" : k.section_axis == 1 ? "y" : "z"));
+  j.key("positions").open('[');
+  for (const double x : k.section_positions) j.val(x);
+  j.close(']').close('}');
   j.key("reference").open('{');
   j.key("length_m").val(d.ref_length).key("temperature_K").val(d.ref_temperature).key("pressure_Pa").val(d.ref_pressure);
   j.key("density_kg_m3").val(d.ref_density).key("velocity_m_s").val(d.ref_velocity);
@@ -586,6 +590,32 @@ int gsk_solver_set_distribution(gsk_solver* sv, const double* f, size_t n) {
     need(sv, "solver");
     need(f, "f");
     sv->sv->set_distribution(f, n);
+  });
+}
+
+int gsk_solver_section_mass_flow(gsk_solver* sv, int32_t axis, const double* positions, size_t n, double* out) {
+  return guarded([&] {
+    need(sv, "solver");
+    if (n == 0) return;
+    need(positions, "positions");
+    need(out, "out");
+    sv->sv->section_mass_flow(axis, std::vector<double>(positions, positions + n), out);
+  });
+}
+
+int gsk_solver_section_cuts(gsk_solver* sv, int32_t axis, const double* positions, size_t n, int64_t* n_faces,
+                            double* area, double* projected_area, double* face_position) {
+  return guarded([&] {
+    need(sv, "solver");
+    if (n == 0) return;
+    need(positions, "positions");
+    const auto& cuts = sv->sv->section_cuts(axis, std::vector<double>(positions, positions + n));
+    for (std::size_t k = 0; k < cuts.size(); ++k) {
+      if (n_faces) n_faces[k] = static_cast<int64_t>(cuts[k].faces.size());
+      if (area) area[k] = cuts[k].area;
+      if (projected_area) projected_area[k] = cuts[k].projected_area;
+      if (face_position) face_position[k] = cuts[k].face_position;
+    }
   });
 }
 

@@ -413,6 +413,37 @@ CaseConfig parse_case_toml(const std::string& text, const std::string& source) {
       cfg.gsis_mass_balance_required = it->second.b;
     }
   }
+  if (doc.tables.count("output")) {  // section positions for the mass-balance record (section_cut.cpp:11-38)
+    const TomlTable& o = doc.tables.at("output");
+    auto ax = o.find("mass_flow_section_axis");
+    if (ax != o.end()) {
+      GSK_CHECK(ax->second.kind == TomlValue::String, "output.mass_flow_section_axis must be a string");
+      const std::string& a = ax->second.s;
+      GSK_CHECK(a == "x" || a == "X" || a == "y" || a == "Y" || a == "z" || a == "Z",
+                "output.mass_flow_section_axis must be \"x\", \"y\" or \"z\" (value: \"" + a + "\")");
+      cfg.section_axis = (a == "x" || a == "X") ? 0 : (a == "y" || a == "Y") ? 1 : 2;
+    }
+    auto ps = o.find("mass_flow_sections");
+    if (ps != o.end()) {
+      GSK_CHECK(ps->second.kind == TomlValue::String, "output.mass_flow_sections must be a string of positions");
+      const std::string& t = ps->second.s;
+      auto sep = [](char c) { return c == ',' || c == ';' || c == ' ' || c == '\t'; };
+      std::size_t i = 0;
+      while (i < t.size()) {
+        while (i < t.size() && sep(t[i])) ++i;
+        if (i >= t.size()) break;
+        std::size_t j = i;
+        while (j < t.size() && !sep(t[j])) ++j;
+        const std::string tok = t.substr(i, j - i);
+        char* end = nullptr;
+        const double v = std::strtod(tok.c_str(), &end);
+        GSK_CHECK(end == tok.c_str() + tok.size() && std::isfinite(v),
+                  "section position \"" + tok + "\" is not a number (list: \"" + t + "\")");
+        cfg.section_positions.push_back(v);
+        i = j;
+      }
+    }
+  }
   if (doc.tables.count("kinetic")) cfg.kinetic = doc.tables.at("kinetic");
   return cfg;
 }

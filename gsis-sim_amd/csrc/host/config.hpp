@@ -70,6 +70,9 @@ struct CaseConfig {
   double gsis_tolerance = 1e-5;
   double gsis_mass_balance_tolerance = 2e-3;  // [gsis].mass_balance_tolerance (cis-run's balance check)
   bool gsis_mass_balance_required = false;    // [gsis].mass_balance_required
+  // [output].mass_flow_sections ("0.2, 0.5 0.8": positions in L_ref units) and mass_flow_section_axis
+  std::vector<double> section_positions;
+  int section_axis = 0;
 
   double gas_constant() const { return kBoltzmann / molecular_mass; }
   double viscosity(double T) const { return mu_ref * std::pow(T / T_ref, omega); }

@@ -223,6 +223,8 @@ KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh,
   s.gsis_tolerance = config.gsis_tolerance;
   s.mass_balance_tolerance = config.gsis_mass_balance_tolerance;
   s.mass_balance_required = config.gsis_mass_balance_required;
+  s.section_positions = config.section_positions;
+  s.section_axis = config.section_axis;
   s.limiter_mode = params.limiter == "none" ? 0 : (params.limiter_eps == "relative" ? 2 : 1);
   s.derived = derive_quantities(config);
   s.length_scale = config.L_ref;

@@ -114,6 +114,8 @@ struct KineticSetup {
   std::string case_name, mesh_file;
   double gsis_tolerance = 1e-5, mass_balance_tolerance = 2e-3;
   bool mass_balance_required = false;
+  std::vector<double> section_positions;  // [output].mass_flow_sections
+  int section_axis = 0;
   int limiter_mode = 1;
   DerivedQuantities derived;
   double length_scale = 1.0;

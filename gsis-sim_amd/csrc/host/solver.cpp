@@ -700,6 +700,84 @@ void KineticSolver::set_newton_tolerances(double cell_tol, double inflow_tol) {
   ++version_;
 }
 
+std::vector<SectionCut> build_section_cuts(const PolyMesh& mesh, const MeshGeometry& geometry, double length_scale,
+                                           int axis, const std::vector<double>& positions) {  // (section_cut.cpp:48-85)
+  GSK_CHECK(axis >= 0 && axis <= 2, "section axis must be 0 (x), 1 (y) or 2 (z)");
+  GSK_CHECK(length_scale > 0.0, "section length scale must be positive");
+  const double L = length_scale;
+  auto coord = [axis](const Vec3& v) { return axis == 0 ? v.x : axis == 1 ? v.y : v.z; };
+  std::vector<SectionCut> cuts;
+  cuts.reserve(positions.size());
+  for (const double X : positions) {
+    GSK_CHECK(std::isfinite(X), "section position is not a finite number");
+    SectionCut cut;
+    cut.position = X;
+    cut.axis = axis;
+    double weighted = 0.0;
+    for (Index f = 0; f < mesh.n_internal_faces(); ++f) {
+      const auto fi = static_cast<std::size_t>(f);
+      const bool o_before = coord(geometry.cell_center[static_cast<std::size_t>(mesh.owner[fi])]) / L < X;
+      const bool n_before = coord(geometry.cell_center[static_cast<std::size_t>(mesh.neighbour[fi])]) / L < X;
+      if (o_before == n_before) continue;
+      const signed char sg = o_before ? 1 : -1;
+      const double A = geometry.face_area[fi] / (L * L);
+      cut.faces.push_back(f);
+      cut.sign.push_back(sg);
+      cut.area += A;
+      cut.projected_area += static_cast<double>(sg) * coord(geometry.face_area_vector[fi]) / (L * L);
+      weighted += A * coord(geometry.face_center[fi]) / L;
+    }
+    cut.face_position = cut.area > 0.0 ? weighted / cut.area : X;
+    cuts.push_back(std::move(cut));
+  }
+  return cuts;
+}
+
+void KineticSolver::build_sections(int axis, const std::vector<double>& positions) {  // (kinetic_solver.cpp:56-77)
+  if (axis == section_axis_ && positions == section_positions_) return;
+  section_cuts_ = build_section_cuts(s_.mesh, s_.geometry, s_.length_scale, axis, positions);
+  section_cells_.clear();
+  section_entries_.clear();
+  for (const SectionCut& cut : section_cuts_) {
+    for (const Index f : cut.faces) {
+      const Index o = s_.mesh.owner[static_cast<std::size_t>(f)];
+      Offset e = -1;
+      for (Offset k = s_.csr.begin(o); k < s_.csr.end(o); ++k)
+        if (s_.csr.faces[static_cast<std::size_t>(k)] == f) {
+          e = k;
+          break;
+        }
+      GSK_CHECK(e >= 0 && s_.csr.sign[static_cast<std::size_t>(e)] > 0,
+                strf("section face %d is not among the faces of its owner %d", static_cast<int>(f), static_cast<int>(o)));
+      section_cells_.push_back(static_cast<int>(o));
+      section_entries_.push_back(static_cast<int>(e));
+    }
+  }
+  section_axis_ = axis;
+  section_positions_ = positions;
+}
+
+const std::vector<SectionCut>& KineticSolver::section_cuts(int axis, const std::vector<double>& positions) {
+  build_sections(axis, positions);
+  return section_cuts_;
+}
+
+void KineticSolver::section_mass_flow(int axis, const std::vector<double>& positions, double* out) {
+  if (positions.empty()) return;
+  prepare();
+  build_sections(axis, positions);
+  std::vector<double> face(section_cells_.size(), 0.0);
+  if (!face.empty())
+    backend_->internal_face_mass_flux(section_cells_.data(), section_entries_.data(), face.size(), face.data());
+  std::size_t k = 0;
+  for (std::size_t c = 0; c < section_cuts_.size(); ++c) {
+    double sum = 0.0;
+    for (std::size_t j = 0; j < section_cuts_[c].faces.size(); ++j, ++k)
+      sum += static_cast<double>(section_cuts_[c].sign[j]) * face[k];
+    out[c] = sum;
+  }
+}
+
 void KineticSolver::residual_per_cell(double* out, std::size_t n) const {
   const auto nc = static_cast<std::size_t>(s_.n_cells());
   GSK_CHECK(n == nc, "residual output length differs from the cell count");

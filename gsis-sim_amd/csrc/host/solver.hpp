@@ -13,6 +13,22 @@
 
 namespace gsk {
 
+// A section normal to an axis at position X (L_ref units): the internal faces between a cell whose
+// centre lies before X and one whose centre does not (src/mesh/section_cut.hpp:20-55). A staircase on
+// a general mesh, a plane on an aligned one; on a steady conservative solution the discrete flux
+// through it equals the inlet / outlet flow. sign = +1 when the face's area vector points forward.
+struct SectionCut {
+  double position = 0.0;
+  int axis = 0;
+  std::vector<Index> faces;  // internal faces of the setup's (renumbered) mesh
+  std::vector<signed char> sign;
+  double area = 0.0, projected_area = 0.0;  // L_ref^2 units; projected = sum of sign * S_axis
+  double face_position = 0.0;               // area-weighted face-centre coordinate (X when empty)
+};
+
+std::vector<SectionCut> build_section_cuts(const PolyMesh& mesh, const MeshGeometry& geometry, double length_scale,
+                                           int axis, const std::vector<double>& positions);
+
 class KineticSolver {
  public:
   // takes ownership of the backend; the setup must outlive the solver
@@ -35,6 +51,10 @@ class KineticSolver {
   void distribution(double* f, std::size_t n);
   void set_distribution(const double* f, std::size_t n);
   void residual_per_cell(double* out, std::size_t n) const;
+  // nondimensional mass flow through each section, along +axis (KineticSolver::section_mass_flow,
+  // kinetic_solver.cpp:669-686); prepares first
+  void section_mass_flow(int axis, const std::vector<double>& positions, double* out);
+  const std::vector<SectionCut>& section_cuts(int axis, const std::vector<double>& positions);
 
   // Device-resident operators for sensitivity solves: arrays are device pointers to
   // [cell][storage position] values in the f precision (internal numbering). Setting f keeps the
@@ -90,6 +110,12 @@ class KineticSolver {
   double mass_target_ = 0.0;
   gsk_inflow_state wall_cache_{};
   std::vector<int> wall_faces_, wall_maxwell_;
+  // section cuts of the last requested (axis, positions) and, per cut face, its owner and CSR entry
+  int section_axis_ = -1;
+  std::vector<double> section_positions_;
+  std::vector<SectionCut> section_cuts_;
+  std::vector<int> section_cells_, section_entries_;
+  void build_sections(int axis, const std::vector<double>& positions);
   std::vector<double> wall_unit0_;
   bool wall_scales_on_device_ = false;  // inflow_ scales of wall faces are behind the backend's
   int wall_cache_patch_ = -1;

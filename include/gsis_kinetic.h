@@ -256,6 +256,16 @@ int gsk_solver_apply_gsis_correction(gsk_solver* sv, const double* w_half, const
 int gsk_solver_distribution(gsk_solver* sv, double* f, size_t n);
 int gsk_solver_set_distribution(gsk_solver* sv, const double* f, size_t n);
 int gsk_solver_residual_per_cell(gsk_solver* sv, double* out, size_t n);
+/* Mass flow through sections normal to axis (0 x, 1 y, 2 z) at n positions in L_ref units: the sum
+ * of the discrete face fluxes over the internal faces between a cell whose centre lies before the
+ * position and one whose centre does not, along +axis, nondimensional (KineticSolver::
+ * section_mass_flow, kinetic_solver.cpp:669-686; build_section_cuts, src/mesh/section_cut.cpp:48-85).
+ * Prepares first. */
+int gsk_solver_section_mass_flow(gsk_solver* sv, int32_t axis, const double* positions, size_t n, double* out);
+/* The same sections' faces: per section the face count, area, projected area (L_ref^2 units) and the
+ * area-weighted face position (KineticSolver::section_cuts); any output may be NULL. */
+int gsk_solver_section_cuts(gsk_solver* sv, int32_t axis, const double* positions, size_t n, int64_t* n_faces,
+                            double* area, double* projected_area, double* face_position);
 
 /* Device-resident operators for sensitivity solves (dMdot/d delta_rp, dMdot/d alpha). No counterpart
  * in the reference; for callers that keep vectors in HBM (the Python sensitivity driver). Arrays

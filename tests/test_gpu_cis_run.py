@@ -69,6 +69,8 @@ def test_cis_run_files_match_oracle(gsk, meshgen, oracle_mod, tmp_path):
     import gsis_sim_amd.cis_run  # noqa: F401
     cr = sys.modules["gsis_sim_amd.cis_run"]
     case = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"})
+    with open(case, "a") as f:  # section flows in both files (src/mesh/section_cut.hpp)
+        f.write('\n[output]\nmass_flow_sections = "0.2, 0.5 0.8"\n')
     cpu = cr.cis_run(case, str(tmp_path / "cpu"), max_iter=6,
                      backend_factory=lambda st: oracle_mod.CpuBackend(st, "oracle", ops_type=gsk.BackendOps))
     gpu = cr.cis_run(case, str(tmp_path / "gpu"), max_iter=6)
