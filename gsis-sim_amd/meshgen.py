@@ -20,19 +20,25 @@ import struct
 
 import numpy as np
 
-def hex_to_poly(points: np.ndarray, cells: np.ndarray, patch_of_face, patch_names, patch_kinds):
-    """Convert hexahedra (8 corners, corner index bits = (i, j, k)) to the .pmesh face layout.
+HEX_FACES = np.array([[0, 2, 6, 4], [1, 3, 7, 5], [0, 1, 5, 4], [2, 3, 7, 6], [0, 1, 3, 2], [4, 5, 7, 6]])
+TET_FACES = np.array([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]])
+
+
+def hex_to_poly(points: np.ndarray, cells: np.ndarray, patch_of_face, patch_names, patch_kinds, faces=HEX_FACES):
+    """Convert cells of one kind to the .pmesh face layout: hexahedra (8 corners, corner index bits =
+    (i, j, k)) by default, or any cell whose faces `faces` lists as cyclic corner sequences of one
+    length (TET_FACES for tetrahedra). Faces are oriented from owner to neighbour / outward.
 
     patch_of_face(centers, normals) -> int patch index per boundary face.
-    Returns dict(points, face_nodes (n_faces, 4), owner, neighbour, patches).
+    Returns dict(points, face_nodes (n_faces, face size), owner, neighbour, patches).
     """
     points = np.asarray(points, np.float64)
     cells = np.asarray(cells, np.int64)
     nc = len(cells)
-    # the 6 faces of every cell as cyclic quads: corners ordered around the face
-    cyc = np.array([[0, 2, 6, 4], [1, 3, 7, 5], [0, 1, 5, 4], [2, 3, 7, 6], [0, 1, 3, 2], [4, 5, 7, 6]])
-    quads = cells[:, cyc].reshape(-1, 4)  # (6 nc, 4)
-    owner_cell = np.repeat(np.arange(nc), 6)
+    cyc = np.asarray(faces)
+    nfc, m = cyc.shape
+    quads = cells[:, cyc].reshape(-1, m)  # (faces per cell x nc, face size)
+    owner_cell = np.repeat(np.arange(nc), nfc)
     key = np.sort(quads, axis=1)
     uniq, inv, counts = np.unique(key, axis=0, return_inverse=True, return_counts=True)
     inv = inv.ravel()
@@ -65,8 +71,8 @@ def hex_to_poly(points: np.ndarray, cells: np.ndarray, patch_of_face, patch_name
         p = points[q]
         xbar = p.mean(axis=1)
         s = np.zeros_like(xbar)
-        for k in range(4):
-            s += np.cross(p[:, k] - xbar, p[:, (k + 1) % 4] - xbar)
+        for k in range(m):
+            s += np.cross(p[:, k] - xbar, p[:, (k + 1) % m] - xbar)
         fc = xbar
         ref = fc - cell_center[c_from] if toward_other is None else cell_center[toward_other] - cell_center[c_from]
         flip = np.einsum("ij,ij->i", s, ref) < 0
@@ -107,11 +113,11 @@ def hex_to_poly(points: np.ndarray, cells: np.ndarray, patch_of_face, patch_name
 
 
 def write_pmesh(path: str, mesh: dict, metadata: dict | None = None) -> None:
-    """Write a .pmesh v1 file (quad faces)."""
+    """Write a .pmesh v1 file (faces of one size: quads or triangles)."""
     pts = np.ascontiguousarray(mesh["points"], "<f8")
     fn = np.ascontiguousarray(mesh["face_nodes"], "<i4")
-    nf = len(fn)
-    fo = np.arange(0, 4 * nf + 1, 4, dtype="<i8")
+    nf, m = fn.shape
+    fo = np.arange(0, m * nf + 1, m, dtype="<i8")
     owner = np.ascontiguousarray(mesh["owner"], "<i4")
     nbr = np.ascontiguousarray(mesh["neighbour"], "<i4")
     arrays = [("points", pts, "float64", [len(pts), 3]), ("face_offsets", fo, "int64", [nf + 1]),
@@ -177,6 +183,25 @@ def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: 
     local spacing (seeded numpy generator; boundary nodes stay on their planes so every boundary
     face keeps an axis-aligned normal). Patches xmin, xmax, ymin, ymax, zmin, zmax, or the six
     `names` in that order (sides with the same name form one patch)."""
+    pts, cells, patch_of, unique = _box_parts(nx, ny, nz, size, distort, seed, names)
+    return hex_to_poly(pts, cells, patch_of, unique, ["patch"] * len(unique))
+
+
+def tet_box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: float = 0.0, seed: int = 0,
+                 names=None) -> dict:
+    """The box of box_mesh with every hexahedron split into six tetrahedra around its (0,0,0)-(1,1,1)
+    diagonal (one per ordering of the axes, so neighbouring hexahedra share their face diagonals)."""
+    pts, hexes, patch_of, unique = _box_parts(nx, ny, nz, size, distort, seed, names)
+    tets = []
+    for a, b in ((0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1)):
+        ba, bb = 1 << a, 1 << b
+        tets.append(hexes[:, [0, ba, ba | bb, 7]])
+    tets = np.stack(tets, axis=1).reshape(-1, 4)
+    return hex_to_poly(pts, tets, patch_of, unique, ["patch"] * len(unique), faces=TET_FACES)
+
+
+def _box_parts(nx, ny, nz, size, distort, seed, names):
+    """Points, hexahedra and the boundary-patch rule of box_mesh."""
     n = (nx, ny, nz)
     axes = [np.linspace(0.0, size[a], n[a] + 1) for a in range(3)]
     I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
@@ -194,7 +219,6 @@ def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: 
         for j in range(ny):
             for k in range(nz):
                 cells.append([nid(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) for c in range(8)])
-
     side_names = list(names) if names is not None else ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
     assert len(side_names) == 6
     unique = list(dict.fromkeys(side_names))
@@ -208,7 +232,7 @@ def box_mesh(nx: int, ny: int, nz: int, size=(1.0e-3, 0.6e-3, 0.4e-3), distort: 
         assert (p >= 0).all()
         return p
 
-    return hex_to_poly(pts, np.array(cells), patch_of, unique, ["patch"] * len(unique))
+    return pts, np.array(cells), patch_of, unique
 
 
 def tube_mesh(n_core: int = 16, n_ring: int = 10, n_axial: int = 45, radius: float = 1.0e-3,

@@ -55,11 +55,12 @@ T0 = 273.15
 
 
 def box_case(mg, directory, n=(5, 3, 2), distort=0.25, seed=7, n_vel=8, vrange=5.0, kinetic=None, bcs="mixed",
-             delta=1.0, name="box", wall_T=(0.95, 1.05), stretch=0.0):
+             delta=1.0, name="box", wall_T=(0.95, 1.05), stretch=0.0, tet=False):
     """Distorted box with every boundary kind the kinetic path handles:
     xmin pressure inlet, xmax pressure outlet (p > 0), ymin moving diffuse wall, ymax Maxwell wall
-    (alpha = 0.8, axis aligned -> mirror shortcut), zmin / zmax symmetry."""
-    mesh = mg.box_mesh(*n, distort=distort, seed=seed)
+    (alpha = 0.8, axis aligned -> mirror shortcut), zmin / zmax symmetry. tet: the same box split
+    into tetrahedra (six per hexahedron)."""
+    mesh = (mg.tet_box_mesh if tet else mg.box_mesh)(*n, distort=distort, seed=seed)
     mg.write_pmesh(os.path.join(directory, f"{name}.pmesh"), mesh)
     if bcs == "mixed":
         bnd = [dict(patch="xmin", type="pressure_inlet", temperature=1.1 * T0),

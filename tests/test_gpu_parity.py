@@ -71,6 +71,10 @@ METHOD_CASES = {
     # tanh-stretched velocity grid (an extension; the oracle carries the same per-velocity weights)
     "box_mixed_fp64_stretched": (dict(kinetic={"precision": "fp64"}, stretch=1.5), 1e-12),
     "box_mixed_fp32_stretched": (dict(kinetic={"precision": "fp32", "sweep_blocks": 2}, stretch=1.5), 2e-6),
+    # the same box split into tetrahedra (four faces per cell: padded slots in the gathered kernels)
+    "tet_box_mixed_fp64": (dict(kinetic={"precision": "fp64", "sweep_blocks": 3}, tet=True), 1e-12),
+    "tet_box_mixed_fp64_positivity": (dict(kinetic={"precision": "fp64", "positivity_limiter": True}, tet=True), 1e-12),
+    "tet_box_mixed_fp32": (dict(kinetic={"precision": "fp32"}, tet=True), 2e-6),
 }
 
 
@@ -118,6 +122,9 @@ SOLVER_CASES = {
     "slab_200_24cubed": lambda mg, d: mg.slab_case(d, kinetic={"precision": "fp64"}),
     # tanh-stretched velocity grid; the curved Maxwell wall interpolates between stretched nodes
     "box_mixed_stretched": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64"}, stretch=1.5),
+    "tet_box_mixed": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64", "sweep_blocks": 3}, tet=True),
+    "tet_box_mixed_positivity": lambda mg, d: box_case(mg, d, kinetic={"precision": "fp64", "positivity_limiter": True},
+                                                       tet=True),
     "tube_curved_maxwell_stretched": lambda mg, d: mg.tube_case(d, n_core=4, n_ring=2, n_axial=3, n_vel=8, vrange=5.0,
                                                                delta=5.0, accommodation=0.9, stretch=1.2,
                                                                kinetic={"precision": "fp64"}),

@@ -69,10 +69,10 @@ def test_boundary_iterations_repeat_the_boundary_update(gsk, meshgen, oracle_mod
     """boundary_iterations = 8 in one prepare equals eight single-iteration prepares separated by a
     zero GSIS correction (which leaves f bit for bit and only invalidates the prepared state), and
     the repeated updates do move the inflow fluxes (the lag they remove is real); restates
-    tests/kinetic/test_kinetic_cpu.cpp:511-560 on a distorted hexahedral box."""
+    tests/kinetic/test_kinetic_cpu.cpp:511-560 on a distorted box of tetrahedra."""
     print(f"[{backend}] boundary iterations", flush=True)
-    mesh = meshgen.box_mesh(3, 2, 2, size=(3.0, 2.0, 2.0), distort=0.2, seed=3,
-                            names=["inlet", "outlet", "wall", "wall", "wall", "wall"])
+    mesh = meshgen.tet_box_mesh(3, 2, 2, size=(3.0, 2.0, 2.0), distort=0.2, seed=3,
+                                names=["inlet", "outlet", "wall", "wall", "wall", "wall"])
     bcs = [dict(patch="inlet", type="pressure_inlet"), dict(patch="outlet", type="vacuum"),
            dict(patch="wall", type="diffuse_wall")]
     solvers, keep = [], []

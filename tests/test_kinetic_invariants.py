@@ -108,3 +108,41 @@ def test_sweep_orderings_converge_to_the_same_solution(gsk, meshgen, oracle_mod,
         assert np.allclose(b[:, 0], a[:, 0], rtol=2e-5, atol=0), spec
         assert np.allclose(b[:, 4], a[:, 4], rtol=2e-5, atol=0), spec
         assert np.max(np.abs(b[:, 1] - a[:, 1])) <= 2e-5, spec
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_positivity_limiter_keeps_boundary_values_non_negative(gsk, meshgen, oracle_mod, tmp_path, backend):
+    """Three cells between vacuum ends holding 1, 0.5 and 0.01 of a Maxwellian: the steep last cell
+    makes the relative-epsilon Venkatakrishnan reconstruction at the right face negative, so its
+    outgoing mass flux F+ is negative; with positivity_limiter it is not (tests/kinetic/
+    test_kinetic_cpu.cpp:393-434)."""
+    print(f"[{backend}] positivity limiter at a vacuum face", flush=True)
+    import math
+
+    mesh = meshgen.box_mesh(3, 1, 1, size=(3.0, 1.0, 1.0), names=["xlo", "xhi", "y", "y", "z", "z"])
+    flux = {}
+    for pos in (False, True):
+        path = kinetic_case(meshgen, str(tmp_path), mesh,
+                            [dict(patch="xlo", type="vacuum"), dict(patch="xhi", type="vacuum"),
+                             dict(patch="y", type="periodic_slab"), dict(patch="z", type="periodic_slab")],
+                            n_vel=12, vmax=6.0, name=f"pos{int(pos)}",
+                            kinetic=dict(precision="fp64", sweep_blocks=1, positivity_limiter=pos))
+        setup = gsk.Setup(path)
+        keep = oracle_mod.CpuBackend(setup, "oracle", ops_type=gsk.BackendOps) if backend == "oracle" else None
+        sv = gsk.Solver(setup, keep.ops) if keep is not None else gsk.Solver(setup)
+        v = setup.view()
+        xs = np.column_stack([np.ctypeslib.as_array(p, (setup.nv,)) for p in (v.xi_x, v.xi_y, v.xi_z)])
+        xi = np.empty_like(xs)
+        xi[setup.storage_to_grid()] = xs
+        feq = (2 * math.pi) ** -1.5 * np.exp(-0.5 * np.sum(xi * xi, axis=1))
+        centers = np.empty((3, 3))
+        centers[setup.cell_new_to_old()] = setup.cell_centers()
+        scale = np.where(centers[:, 0] < 1.0, 1.0, np.where(centers[:, 0] < 2.0, 0.5, 0.01))
+        sv.set_distribution(scale[:, None] * feq[None, :])
+        sv.prepare()
+        fp, _, _ = sv.boundary_fluxes()
+        patch, _, _ = setup.bface_geometry()
+        flux[pos] = float(fp[np.flatnonzero(patch == 1)[0], 0])  # the xhi face
+        sv.close()
+    assert flux[False] < 0.0
+    assert flux[True] >= -1e-12 * abs(flux[False])
