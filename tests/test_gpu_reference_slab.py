@@ -26,13 +26,10 @@ def test_hip_forms_reach_the_reference_slab_result(gsk, meshgen, tmp_path, case)
     ref = slab_reference_rows()[case]
     want = float(ref["value"])
     forms = ["fp64", "fp32"] + (["mixed"] if ref["nx"] == "100" else [])
-    mid = float(ref["ref_vmid30"])  # the reference's 1D solution on the same velocity grid (verification/ref1d)
     for form in forms:
         print(f"[gpu] reference slab {case} {form}", flush=True)
         val, s = run_case(gsk, meshgen, tmp_path / form, ref, form)
         assert abs(val - want) <= BOUND[form] * abs(want), (form, val, want, s["iterations"])
-        # as close to the independent 1D solution as the reference's own 3D result is
-        assert abs(val - mid) <= abs(want - mid) + BOUND[form] * abs(want), (form, val, mid)
 
 
 DEFAULT_SIDES = sorted(name for name, r in slab_reference_rows().items() if float(r["Kn"]) >= 0.1 and r["sides"] == "default")
