@@ -1,0 +1,179 @@
+"""Per-point functions the kernels use (gsis-sim_amd/csrc/kernels/kinetic_math.hpp), compiled on the
+host (the header is plain C++ outside hipcc) and checked against what the mathematics and the
+reference's own tests fix:
+
+- the Venkatakrishnan limiter's known values (tests/test_physics.cpp:174-200): 1 at a zero
+  projection, 0 at a zero-width extremum without epsilon, 3/4 when the projection equals the
+  neighbour maximum (either side, FP64 and FP32), above 0.99 for a large epsilon;
+- the kernels' rewrite of phi = min over faces of venkat(proj_k) as min(venkat(max proj),
+  venkat(min proj)) (limit_from_extremes in kinetic_view.hpp): bit-exact on 200k random face sets
+  per precision, and within 4 ulp when the projections lie a few ulps apart, where the floating-
+  point limiter is not monotone (measured up to 2.6 ulp);
+- the Shakhov target (make_cell_params + target_eval) against its closed-form moments, and the
+  cold-BGK switch."""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KERNELS = os.path.join(os.path.dirname(HERE), "gsis-sim_amd", "csrc", "kernels")
+
+HARNESS = r"""
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include "kinetic_math.hpp"
+using namespace gsk::dev;
+
+template <class F>
+void trial(long long n, unsigned seed, bool clustered, long long& bad, double& worst) {
+  std::mt19937_64 g(seed);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  bad = 0;
+  worst = 0.0;
+  for (long long t = 0; t < n; ++t) {
+    const F dmax = F(std::pow(10.0, -3 + 4 * u(g))), dmin = -F(std::pow(10.0, -3 + 4 * u(g)));
+    const F eps2 = u(g) < 0.25 ? F(0) : F(std::pow(10.0, -8 + 8 * u(g)));
+    const int nf = clustered ? 6 : 1 + int(6 * u(g));
+    const F base = F((u(g) < 0.5 ? -1 : 1) * std::pow(10.0, -4 + 5 * u(g)));
+    F mn = F(1), pmax = F(0), pmin = F(0), p = base;
+    for (int k = 0; k < nf; ++k) {
+      if (!clustered) p = u(g) < 0.1 ? F(0) : F((u(g) < 0.5 ? -1 : 1) * std::pow(10.0, -4 + 5 * u(g)));
+      const F v = venkat_face(p, dmax, dmin, eps2);
+      mn = v < mn ? v : mn;
+      pmax = pmax < p ? p : pmax;
+      pmin = p < pmin ? p : pmin;
+      if (clustered) p = std::nextafter(p, base > 0 ? F(INFINITY) : F(-INFINITY));
+    }
+    const F a = venkat_face(pmax, dmax, dmin, eps2), b = venkat_face(pmin, dmax, dmin, eps2);
+    const F two = b < a ? b : a;
+    if (std::memcmp(&two, &mn, sizeof(F)) != 0) {
+      ++bad;
+      const double r = std::fabs(double(two) - double(mn)) / std::fabs(double(mn));
+      worst = r > worst ? r : worst;
+    }
+  }
+}
+
+int main() {
+  std::printf("%.17g %.17g %.17g %.17g %.17g %.9g\n", venkat_face(0.0, 1.0, -1.0, 0.0), venkat_face(0.5, 0.0, -1.0, 0.0),
+              venkat_face(1.0, 1.0, -1.0, 0.0), venkat_face(-1.0, 1.0, -1.0, 0.0), venkat_face(1.0, 1.0, -1.0, 100.0),
+              double(venkat_face(1.0f, 1.0f, -1.0f, 0.0f)));
+  long long b;
+  double w;
+  trial<double>(200000, 1, false, b, w);
+  std::printf("%lld %.17g\n", b, w);
+  trial<float>(200000, 2, false, b, w);
+  std::printf("%lld %.17g\n", b, w);
+  trial<double>(200000, 3, true, b, w);
+  std::printf("%lld %.17g\n", b, w);
+  trial<float>(200000, 4, true, b, w);
+  std::printf("%lld %.17g\n", b, w);
+}
+"""
+
+
+def test_limiter_values_and_two_extreme_rewrite(tmp_path):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available to compile the limiter harness")
+    src = tmp_path / "lim.cpp"
+    src.write_text(HARNESS)
+    exe = tmp_path / "lim"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", f"-I{KERNELS}", str(src), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    v = [float(x) for x in lines[0].split()]
+    assert v[0] == 1.0 and v[1] == 0.0
+    assert v[2] == pytest.approx(0.75, abs=1e-15) and v[3] == pytest.approx(0.75, abs=1e-15)
+    assert v[4] > 0.99 and v[5] == pytest.approx(0.75, abs=1e-7)
+    sep64, sep32, cl64, cl32 = ([float(x) for x in ln.split()] for ln in lines[1:5])
+    assert sep64[0] == 0 and sep32[0] == 0  # separated projections: bit for bit
+    assert cl64[1] <= 4 * 2.0 ** -52 and cl32[1] <= 4 * 2.0 ** -23  # ulp-clustered: rounding only
+
+
+SHAKHOV = r"""
+#include <cmath>
+#include <cstdio>
+#include "kinetic_math.hpp"
+using namespace gsk::dev;
+
+int main() {
+  // Shakhov target of (rho, u, T, q) with Pr = 2/3 on 48^3 points over [-9, 9]^3, moments in FP64
+  double m[kMoments] = {};
+  m[0] = 1.2; m[1] = 0.3; m[2] = -0.2; m[3] = 0.5; m[4] = 0.9; m[5] = m[0] * m[4];
+  m[12] = 0.08; m[13] = -0.05; m[14] = 0.02;
+  const CellP<double> cp = make_cell_params<double>(m, 1.0, 0.0, 1e-3, 0.66, 2.0 / 3.0, 0.0);
+  const int n = 48;
+  const double h = 18.0 / n, w = h * h * h;
+  double s0 = 0, s1[3] = {0, 0, 0}, s2[3][3] = {{0}}, s3[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
+        const double x[3] = {-9 + (i + 0.5) * h, -9 + (j + 0.5) * h, -9 + (k + 0.5) * h};
+        const double f = target_eval(cp.s, x[0], x[1], x[2]) * w;
+        s0 += f;
+        for (int a = 0; a < 3; ++a) s1[a] += f * x[a];
+      }
+  const double u[3] = {s1[0] / s0, s1[1] / s0, s1[2] / s0};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
+        const double x[3] = {-9 + (i + 0.5) * h, -9 + (j + 0.5) * h, -9 + (k + 0.5) * h};
+        const double f = target_eval(cp.s, x[0], x[1], x[2]) * w;
+        const double c[3] = {x[0] - u[0], x[1] - u[1], x[2] - u[2]};
+        const double c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+        for (int a = 0; a < 3; ++a) {
+          s3[a] += 0.5 * f * c[a] * c2;
+          for (int b = 0; b < 3; ++b) s2[a][b] += f * c[a] * c[b];
+        }
+      }
+  const double T = (s2[0][0] + s2[1][1] + s2[2][2]) / (3.0 * s0);
+  std::printf("%.17g %.17g %.17g %.17g %.17g\n", s0, u[0], u[1], u[2], T);
+  const double p = s0 * T;
+  std::printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", s2[0][0] - p, s2[1][1] - p, s2[2][2] - p, s2[0][1], s2[1][2], s2[0][2]);
+  std::printf("%.17g %.17g %.17g\n", s3[0], s3[1], s3[2]);
+  // q = 0: the target is the Maxwellian
+  double m0[kMoments] = {};
+  m0[0] = 1.2; m0[1] = 0.3; m0[2] = -0.2; m0[3] = 0.5; m0[4] = 0.9; m0[5] = m0[0] * m0[4];
+  const CellP<double> c0 = make_cell_params<double>(m0, 1.0, 0.0, 1e-3, 0.66, 2.0 / 3.0, 0.0);
+  const double xi[3] = {0.7, -1.1, 0.4};
+  std::printf("%.17g %.17g\n", target_eval(c0.s, xi[0], xi[1], xi[2]), maxw_eval(c0.s.eq, xi[0], xi[1], xi[2]));
+  // cold_bgk_temperature (tests/kinetic/test_kinetic_phase5.cpp:373-388)
+  double mc[kMoments] = {};
+  mc[0] = 0.02; mc[1] = 2.0; mc[4] = 0.05; mc[5] = mc[0] * mc[4]; mc[12] = 1e-4;
+  const CellP<double> warm = make_cell_params<double>(mc, 1.0, 0.0, 1e-3, 0.66, 2.0 / 3.0, 0.0);
+  const CellP<double> cold = make_cell_params<double>(mc, 1.0, 0.0, 1e-3, 0.66, 2.0 / 3.0, 0.1);
+  const CellP<double> above = make_cell_params<double>(mc, 1.0, 0.0, 1e-3, 0.66, 2.0 / 3.0, 0.04);
+  std::printf("%.17g %.17g %.17g %.17g %.17g\n", warm.s.coef, cold.s.coef, above.s.coef, cold.coll, warm.coll);
+}
+"""
+
+
+def test_shakhov_target_moments_and_cold_bgk_switch(tmp_path):
+    """The target the kernels evaluate (make_cell_params + target_eval): on 48^3 points over [-9, 9]^3
+    its moments are (rho, u, T) with no stress and heat flux (1 - Pr) q (1e-10); with q = 0 it is the
+    Maxwellian; below cold_bgk_temperature its Shakhov coefficient is 0 and the collision frequency
+    unchanged (tests/test_physics.cpp:126-145, tests/kinetic/test_kinetic_phase5.cpp:373-388)."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available to compile the harness")
+    src = tmp_path / "shakhov.cpp"
+    src.write_text(SHAKHOV)
+    exe = tmp_path / "shakhov"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", f"-I{KERNELS}", str(src), "-o", str(exe)], check=True)
+    rows = [[float(x) for x in ln.split()] for ln in
+            subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.strip().split("\n")]
+    rho, ux, uy, uz, T = rows[0]
+    assert abs(rho / 1.2 - 1) < 1e-10 and abs(T / 0.9 - 1) < 1e-10
+    assert max(abs(ux - 0.3), abs(uy + 0.2), abs(uz - 0.5)) < 1e-10
+    assert max(abs(x) for x in rows[1]) < 1e-10
+    q = (1 - 2.0 / 3.0) * np.array([0.08, -0.05, 0.02])
+    assert np.max(np.abs(np.array(rows[2]) - q)) < 1e-10
+    assert rows[3][0] == pytest.approx(rows[3][1], rel=1e-15)
+    warm, cold, above, coll_cold, coll_warm = rows[4]
+    assert warm > 0.0 and cold == 0.0 and above == warm and coll_cold == coll_warm
