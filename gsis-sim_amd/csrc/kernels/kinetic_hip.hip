@@ -945,6 +945,9 @@ __device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z
   const int n_in = uniform(r.n_in), n_io = n_in + uniform(r.n_out);
   const i64 row = uniform64(r.row_w);
   const F own = row_load(byte_row(cur, row), toff);
+  // the cell's f is read with its own value, so that load overlaps the neighbour loads instead of
+  // following the store of df (no other cell writes this (cell, velocity) during the sweep)
+  const F f_old = update_f ? row_load(byte_row(static_cast<const F*>(f), uniform64(r.row_f)), poff) : F(0);
   F am[kGather], nb[kGather];  // as in forward_rec
   F ap = F(0);
 #pragma unroll
@@ -964,7 +967,7 @@ __device__ __forceinline__ void backward_rec(const SweepRec<F>& r, F x, F y, F z
   const F D = madd(madd(r.diag, diag_scale, ap), damping, am_out);
   const F df = own - sweep_div(U, D);
   row_ref(byte_row(cur, row), toff) = df;
-  if (update_f) row_ref(byte_row(f, uniform64(r.row_f)), poff) += df;
+  if (update_f) row_ref(byte_row(f, uniform64(r.row_f)), poff) = f_old + df;
 }
 
 // Workgroup -> (direction, block, velocity tile). The (direction, block) pairs are dealt to the 8
