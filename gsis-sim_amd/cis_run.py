@@ -17,8 +17,12 @@ as the reference writes them (history columns M_dot_sec<k>, summary mass_flow_se
 balance record's section spread). Not written: fields.vtu / boundary.vtu (visualisation) and the
 STOP-file interrupt. cells.csv is written when asked (`cells_csv=True`, the reference's --cells-csv).
 
+`init_from` restarts from a previous cells.csv: f = the equilibrium of each cell's (rho, u, T)
+(read_kinetic_cells_csv, src/kinetic/kinetic_output.cpp:142-196; cmd_cis_run.cpp:215-219).
+
 Run as a script: python gsis-sim_amd/cis_run.py case.toml [--out DIR] [--max-iter N] [--min-iter N]
 [--tol X] [--precision fp32|fp64] [--set key=value]... [--print-every N] [--cells-csv]
+[--init-from cells.csv]
 """
 
 from __future__ import annotations
@@ -386,6 +390,36 @@ def sections_record(desc: dict, cuts: dict, flows) -> dict:
             "sections": out}
 
 
+def read_cells_csv(path: str, n_cells: int) -> np.ndarray:
+    """(rho, ux, uy, uz, T) per cell from a cells.csv, located by the header's cell, rho, ux, uy, uz
+    and T columns; every cell must appear (read_kinetic_cells_csv, kinetic_output.cpp:142-196)."""
+    import csv
+
+    with open(path, newline="") as f:
+        rows = list(csv.reader(f))
+    if not rows:
+        raise ValueError(f"{path} is empty")
+    head = rows[0]
+    need = ("cell", "rho", "ux", "uy", "uz", "T")
+    missing = [c for c in need if c not in head]
+    if missing:
+        raise ValueError(f"{path} has no column(s) {', '.join(missing)}")
+    col = [head.index(c) for c in need]
+    w = np.zeros((n_cells, 5))
+    seen = np.zeros(n_cells, bool)
+    for r in rows[1:]:
+        if not r:
+            continue
+        c = int(float(r[col[0]]))
+        if not 0 <= c < n_cells:
+            raise ValueError(f"{path}: cell number {c} is out of range")
+        w[c] = [float(r[k]) for k in col[1:]]
+        seen[c] = True
+    if not seen.all():
+        raise ValueError(f"{path} has no values for cell {int(np.flatnonzero(~seen)[0])}")
+    return w
+
+
 def write_cells_csv(path: str, setup: Setup, solver: Solver, desc: dict) -> None:
     """cells.csv (write_kinetic_cells_csv, kinetic_output.cpp:125-140): centre / L, V / L^3 and the
     15 nondimensional cell values per cell, original numbering, plus the cell residual."""
@@ -412,7 +446,8 @@ def write_cells_csv(path: str, setup: Setup, solver: Solver, desc: dict) -> None
 
 def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = None, min_iter: int = 1,
             tol: float | None = None, precision: str | None = None, overrides: dict | None = None,
-            print_every: int = 0, cells_csv: bool = False, backend_factory=None, log=print) -> dict:
+            print_every: int = 0, cells_csv: bool = False, backend_factory=None, log=print,
+            init_from: str | None = None) -> dict:
     """CIS iterations to eq. (59) convergence (or max_iter); writes history.csv and summary.json
     into out_dir (default runs/kinetic/<case name>) and returns the summary. backend_factory(setup)
     gives an external backend (an ops table, or an object holding one as .ops, kept alive for the
@@ -436,8 +471,11 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     tol = float(tol if tol is not None else desc["tolerance"])
     out_dir = out_dir or os.path.join("runs", "kinetic", desc["case"])
     os.makedirs(out_dir, exist_ok=True)
-    solver.initialize_uniform(params["initial_pressure"] / params["initial_temperature"], (0.0, 0.0, 0.0),
-                              params["initial_temperature"])
+    if init_from:
+        solver.initialize_equilibrium(read_cells_csv(init_from, setup.n_cells))
+    else:
+        solver.initialize_uniform(params["initial_pressure"] / params["initial_temperature"], (0.0, 0.0, 0.0),
+                                  params["initial_temperature"])
     backend = solver.backend()
     memory_bytes = backend.memory_bytes()
     history = HistoryWriter(os.path.join(out_dir, "history.csv"), desc,
@@ -594,6 +632,7 @@ def main(argv=None) -> int:
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE", help="[kinetic] override")
     ap.add_argument("--print-every", type=int, default=100)
     ap.add_argument("--cells-csv", action="store_true")
+    ap.add_argument("--init-from", help="restart from the cell values of a previous cells.csv")
     a = ap.parse_args(argv)
     ov = {}
     for kv in a.set:
@@ -601,7 +640,8 @@ def main(argv=None) -> int:
         if not k or not _:
             ap.error(f"--set expects key=value, got {kv!r}")
         ov[k.strip()] = _parse_value(v.strip())
-    s = cis_run(a.case, a.out, a.max_iter, a.min_iter, a.tol, a.precision, ov, a.print_every, a.cells_csv)
+    s = cis_run(a.case, a.out, a.max_iter, a.min_iter, a.tol, a.precision, ov, a.print_every, a.cells_csv,
+                init_from=a.init_from)
     print(f"iterations {s['iterations']} ({'converged' if s['converged'] else 'limit reached'}), "
           f"{s['wall_time_s']:.2f} s")
     return 0

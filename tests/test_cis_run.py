@@ -192,3 +192,31 @@ def test_in_memory_loop_matches_the_files(gsk, meshgen, oracle_mod, cr, tmp_path
             "iteration": np.arange(5)}
     assert cr.time_to_within(fake, 1.0, 1e-4) == (3.0, 2)
     assert cr.time_to_within({**fake, "mdot_in": np.array([1.0, 1.0, 1.0, 1.0, 2.0])}, 1.0, 1e-4) is None
+
+
+def test_restart_from_cells_csv(gsk, meshgen, oracle_mod, cr, tmp_path):
+    """--init-from: a run restarted from cells.csv starts from the equilibrium of those cell values
+    (the discrete equilibrium reproduces them to round-off), and a cells.csv that lacks a cell or a
+    column is rejected."""
+    case = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"})
+    fac = lambda st: oracle_mod.CpuBackend(st, "oracle", ops_type=gsk.BackendOps)  # noqa: E731
+    _run(gsk, cr, oracle_mod, case, str(tmp_path / "a"), max_iter=3, cells_csv=True)
+    cells = str(tmp_path / "a" / "cells.csv")
+    w = cr.read_cells_csv(cells, 30)
+    setup = cr.Setup(case)
+    keep = fac(setup)
+    sv = cr.Solver(setup, keep.ops)
+    sv.initialize_equilibrium(w)
+    m = sv.moments()
+    assert np.allclose(m[:, [0, 4]], w[:, [0, 4]], rtol=1e-10, atol=0)
+    assert np.max(np.abs(m[:, 1:4] - w[:, 1:4])) < 1e-10
+    sv.close()
+    s = cr.cis_run(case, str(tmp_path / "b"), max_iter=1, backend_factory=fac, init_from=cells)
+    assert s["iterations"] == 1
+    rows = open(cells).read().splitlines()
+    open(str(tmp_path / "short.csv"), "w").write("\n".join(rows[:-1]) + "\n")
+    with pytest.raises(ValueError, match="no values for cell"):
+        cr.read_cells_csv(str(tmp_path / "short.csv"), 30)
+    open(str(tmp_path / "nocol.csv"), "w").write(rows[0].replace(",T,", ",Temp,") + "\n")
+    with pytest.raises(ValueError, match="no column"):
+        cr.read_cells_csv(str(tmp_path / "nocol.csv"), 30)
