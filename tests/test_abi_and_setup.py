@@ -181,3 +181,31 @@ def test_velocity_grid_stretch_is_validated(gsk, meshgen, tmp_path):
     open(bad, "w").write(text.replace("stretch = 1.5", "stretch = -1.0"))
     with pytest.raises(gsk.GskError, match="stretch must be in"):
         gsk.Setup(bad)
+
+
+@pytest.mark.parametrize("order,n_dirs", [("octant", 4), ("dir13", 13), ("block", 0)])
+def test_sweep_directions_group_the_velocities(gsk, meshgen, tmp_path, order, n_dirs):
+    """make_sweep_directions / sweep_direction_of / make_velocity_layout (the reference's
+    tests/kinetic/test_kinetic_phase5.cpp:241-264): octant gives 4 directions, dir13 13, block none;
+    xi and -xi share a direction while flipping one component changes it (octant); without mirror
+    axes (walls only) the storage order runs through the directions in order."""
+    from conftest import kinetic_case
+
+    mesh = meshgen.box_mesh(2, 2, 2, size=(1.0, 1.0, 1.0), names=["wall"] * 6)
+    path = kinetic_case(meshgen, str(tmp_path), mesh, [dict(patch="wall", type="diffuse_wall")], n_vel=6, vmax=3.0,
+                        kinetic=dict(precision="fp64", sweep_order=order), name=f"dirs_{order}")
+    s = gsk.Setup(path)
+    v = s.view()
+    assert v.n_sweep_dirs == n_dirs
+    if n_dirs == 0:
+        return
+    nv = s.nv
+    xi = np.column_stack([np.ctypeslib.as_array(p, (nv,)) for p in (v.xi_x, v.xi_y, v.xi_z)])
+    d = np.ctypeslib.as_array(v.sweep_dir_of, (nv,)).astype(int)
+    assert np.all(np.diff(d) >= 0) and len(np.unique(d)) == n_dirs  # grouped, every direction used
+    index = {tuple(x): i for i, x in enumerate(xi)}
+    for p in range(nv):
+        assert d[index[tuple(-xi[p])]] == d[p]  # xi and -xi: one direction
+        if order == "octant":
+            flipped = xi[p] * np.array([1.0, -1.0, 1.0])
+            assert d[index[tuple(flipped)]] != d[p]
