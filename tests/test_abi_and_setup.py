@@ -209,3 +209,34 @@ def test_sweep_directions_group_the_velocities(gsk, meshgen, tmp_path, order, n_
         if order == "octant":
             flipped = xi[p] * np.array([1.0, -1.0, 1.0])
             assert d[index[tuple(flipped)]] != d[p]
+
+
+def test_kinetic_parameters_defaults_values_and_errors(gsk, meshgen, tmp_path):
+    """[kinetic] defaults, values and rejections as the reference's tests/kinetic/test_kinetic_params.cpp
+    :44-86 state them, read back through the case description."""
+    from conftest import kinetic_case
+
+    mesh = meshgen.box_mesh(2, 1, 1, size=(1.0, 0.5, 0.5), names=["wall"] * 6)
+
+    def params(kin, name):
+        path = kinetic_case(meshgen, str(tmp_path), mesh, [dict(patch="wall", type="diffuse_wall")], kinetic=kin,
+                            name=name)
+        return gsk.Setup(path).describe()
+
+    d = params(None, "defaults")
+    assert d["params"]["pseudo_cfl"] == 0.0 and d["params"]["precision"] == "fp32"
+    assert d["params"]["boundary_iterations"] == 1
+    p = params(dict(pseudo_cfl=50.0, sweep_blocks=8, precision="fp64", backend="cpu"), "values")
+    assert p["params"]["pseudo_cfl"] == 50.0 and p["params"]["precision"] == "fp64"
+    assert p["params"]["backend"] == "cpu" and p["sweep_blocks"] == 2  # 8 requested, at most one block per cell
+    assert params(dict(boundary_iterations=8), "bi")["params"]["boundary_iterations"] == 8
+    q = params(dict(limiter="none", limiter_eps="relative", initial_pressure=0.5, initial_temperature=1.2,
+                    convergence_u_floor=0.01, launch_cells=100, sweep_cells_per_launch=10), "more")["params"]
+    assert q["limiter"] == "none" and q["limiter_eps"] == "relative" and q["initial_pressure"] == 0.5
+    assert q["initial_temperature"] == 1.2 and q["convergence_u_floor"] == 0.01
+    for kin, word in ((dict(boundary_iterations=0), "boundary_iterations"),
+                      (dict(boundary_iterations=1001), "boundary_iterations"), (dict(sweep_block=8), "sweep_block"),
+                      (dict(precision="fp16"), "kinetic.precision"), (dict(limiter="minmod"), "kinetic.limiter"),
+                      (dict(initial_pressure=0.0), "initial_pressure")):
+        with pytest.raises(gsk.GskError, match=word):
+            params(kin, "bad")
