@@ -22,9 +22,11 @@
 // interpolation brackets the reflected velocity between stretched nodes. The reference has no such
 // grid, so these paths are pinned by closed forms instead (tests/test_stretched_grid.py: Gaussian
 // quadrature of the weights, Maxwellian moments to 1e-10, exact conservation of the discrete
-// equilibrium, equilibrium preservation). The Maxwell-wall interpolation on a stretched grid has no
-// closed-form check: parity unpinned (the GPU matches it, which pins only their agreement). On the
-// uniform grid both pointers are NULL and every path is the reference's, bit for bit.
+// equilibrium, equilibrium preservation). The Maxwell-wall interpolation on a stretched grid is
+// pinned through the kernels' reflected_value, compiled on the host and checked against the closed
+// form M_eq(r) g(r) for f = M_eq g with g linear (tests/test_kernel_math.py); this restatement
+// matches the kernels to 1e-10 over whole CIS iterations on a stretched-grid curved Maxwell tube. On the uniform
+// grid both pointers are NULL and every path is the reference's, bit for bit.
 
 #include <omp.h>
 

@@ -10,7 +10,9 @@ reference's own tests fix:
   per precision, and within 4 ulp when the projections lie a few ulps apart, where the floating-
   point limiter is not monotone (measured up to 2.6 ulp);
 - the Shakhov target (make_cell_params + target_eval) against its closed-form moments, and the
-  cold-BGK switch."""
+  cold-BGK switch;
+- the curved Maxwell wall's reflected value (reflected_value in kinetic_view.hpp) against the closed
+  form it must reproduce when f / M_eq is linear in the velocity, on the stretched and uniform grids."""
 
 from __future__ import annotations
 
@@ -177,3 +179,147 @@ def test_shakhov_target_moments_and_cold_bgk_switch(tmp_path):
     assert rows[3][0] == pytest.approx(rows[3][1], rel=1e-15)
     warm, cold, above, coll_cold, coll_warm = rows[4]
     assert warm > 0.0 and cold == 0.0 and above == warm and coll_cold == coll_warm
+
+
+REFLECTION = r"""
+#include <cmath>
+#include <cstdio>
+#include <vector>
+#define __device__
+#define __forceinline__ inline
+#include "kinetic_view.hpp"
+using namespace gsk::dev;
+
+// g = f / M_eq is linear in the velocity, so the kernel's trilinear interpolation of g between the
+// bracketing nodes must return M_eq(r) g(r) at the reflected velocity r, up to rounding.
+static double lin(const double* r) { return 3.0 + 0.3 * r[0] - 0.2 * r[1] + 0.25 * r[2]; }
+
+template <class F>
+void run(bool stretched) {
+  const int n[3] = {24, 20, 16};
+  const double h = 6.0, b = 1.5;
+  std::vector<double> nodes;
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < n[a]; ++i) {
+      const double s = -1.0 + (i + 0.5) * 2.0 / n[a];
+      nodes.push_back(stretched ? (h / b) * std::atanh(s * std::tanh(b)) : -h + (i + 0.5) * 2.0 * h / n[a]);
+    }
+  const int nv = n[0] * n[1] * n[2];
+  std::vector<F> xs(nv), ys(nv), zs(nv), f(nv);
+  std::vector<int> g2s(nv);
+  const double ux = 0.3, uy = -0.2, uz = 0.1, B = 1.0 / (2.0 * 1.1);
+  double scale = 0.0;
+  for (int iz = 0; iz < n[2]; ++iz)
+    for (int iy = 0; iy < n[1]; ++iy)
+      for (int ix = 0; ix < n[0]; ++ix) {
+        const int g = ix + n[0] * (iy + n[1] * iz);
+        const int q = nv - 1 - g;  // storage order reversed, so the grid-to-storage map is exercised
+        g2s[g] = q;
+        xs[q] = F(nodes[ix]);
+        ys[q] = F(nodes[n[0] + iy]);
+        zs[q] = F(nodes[n[0] + n[1] + iz]);
+        const double r[3] = {double(xs[q]), double(ys[q]), double(zs[q])};
+        const double c2 = (r[0] - ux) * (r[0] - ux) + (r[1] - uy) * (r[1] - uy) + (r[2] - uz) * (r[2] - uz);
+        f[q] = F(std::exp(-B * c2) * lin(r));
+        scale = std::fmax(scale, std::fabs(double(f[q])));
+      }
+  CellP<F> cell{};
+  cell.s.eq = Maxw<F>{F(ux), F(uy), F(uz), F(1), F(B)};
+  View<F> v{};
+  v.nv = nv;
+  v.xi_x = xs.data();
+  v.xi_y = ys.data();
+  v.xi_z = zs.data();
+  v.nodes = stretched ? nodes.data() : nullptr;
+  v.cells = &cell;
+  v.f = f.data();
+  v.grid_to_storage = g2s.data();
+  for (int a = 0; a < 3; ++a) {
+    v.grid_n[a] = n[a];
+    v.deq.lo[a] = -h;
+    v.deq.d[a] = 2.0 * h / n[a];
+  }
+  double worst = 0.0;
+  long inside = 0, outside = 0, outside_nonzero = 0, hit_nodes = 0;
+  unsigned long long state = 12345;
+  for (int k = 0; k < 40; ++k) {
+    double nrm[3];
+    if (k == 0) {
+      nrm[0] = nrm[1] = 1.0 / std::sqrt(2.0), nrm[2] = 0.0;  // x <-> y swap: nodes onto nodes (x, y share no count)
+    } else {
+      double l = 0.0;
+      for (double& c : nrm) {
+        state = state * 6364136223846793005ULL + 1442695040888963407ULL;
+        c = double(state >> 11) / 9007199254740992.0 - 0.5;
+        l += c * c;
+      }
+      for (double& c : nrm) c /= std::sqrt(l);
+    }
+    BFace<F> bf{};
+    bf.nx = F(nrm[0]);
+    bf.ny = F(nrm[1]);
+    bf.nz = F(nrm[2]);
+    for (int p = 0; p < nv; ++p) {
+      const F x = xs[p], y = ys[p], z = zs[p];
+      if (!(x * bf.nx + y * bf.ny + z * bf.nz < F(0))) continue;
+      const F got = reflected_value<false, F>(v, bf, p, x, y, z);
+      const double un = double(x) * double(bf.nx) + double(y) * double(bf.ny) + double(z) * double(bf.nz);
+      const double r[3] = {double(x) - 2.0 * un * double(bf.nx), double(y) - 2.0 * un * double(bf.ny),
+                           double(z) - 2.0 * un * double(bf.nz)};
+      int where = 0;  // 0 inside, 1 clearly outside, 2 within 1e-3 of the end spacing of the edge
+      int off = 0;
+      for (int a = 0; a < 3; ++a) {
+        const double* nd = nodes.data() + off;
+        const double lo = nd[0], hi = nd[n[a] - 1], d = nd[1] - nd[0];
+        off += n[a];
+        if (r[a] < lo - 1e-3 * d || r[a] > hi + 1e-3 * d)
+          where = 1;
+        else if (where == 0 && (r[a] < lo + 1e-3 * d || r[a] > hi - 1e-3 * d))
+          where = 2;
+      }
+      if (where == 1) {
+        ++outside;
+        if (got != F(0)) ++outside_nonzero;
+        continue;
+      }
+      if (where == 2) continue;
+      ++inside;
+      const double c2 = (r[0] - ux) * (r[0] - ux) + (r[1] - uy) * (r[1] - uy) + (r[2] - uz) * (r[2] - uz);
+      const double expect = std::exp(-B * c2) * lin(r);
+      worst = std::fmax(worst, std::fabs(double(got) - expect) / scale);
+      if (k == 0) ++hit_nodes;
+    }
+  }
+  std::printf("%.6g %ld %ld %ld %ld\n", worst, inside, outside, outside_nonzero, hit_nodes);
+}
+
+int main() {
+  run<double>(true);
+  run<double>(false);
+  run<float>(true);
+  run<float>(false);
+}
+"""
+
+
+def test_maxwell_wall_reflection_interpolates_exactly_for_linear_ratio(tmp_path):
+    """The curved Maxwell wall's reflected value (reflected_value in kinetic_view.hpp, the function
+    the reflection kernel runs; kinetic_kernels.hpp:80-127 on the uniform grid) interpolates
+    g = f / M_eq trilinearly between the bracketing nodes and multiplies by M_eq at the reflected
+    velocity r. For f = M_eq times a linear function that is exact, so the result must be
+    M_eq(r) g(r): on the tanh-stretched grid (24 x 20 x 16 nodes on [-6, 6], stretch 1.5, where
+    the bracketing is a bisection over the node table) and on the uniform grid, for 40 wall normals
+    including the x-y diagonal, with storage order reversed against grid order; and every reflected
+    velocity outside the node range gives 0. A wrong bracket, axis, weight or Maxwellian ratio
+    breaks the equality by far more than rounding."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available to compile the harness")
+    src = tmp_path / "reflect.cpp"
+    src.write_text(REFLECTION)
+    exe = tmp_path / "reflect"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", f"-I{KERNELS}", str(src), "-o", str(exe)], check=True)
+    rows = [[float(x) for x in ln.split()] for ln in
+            subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.strip().split("\n")]
+    for (worst, inside, outside, outside_nonzero, diagonal), bound in zip(rows, (1e-13, 1e-13, 2e-6, 2e-6)):
+        assert worst < bound, (worst, bound)
+        assert inside > 100000 and outside > 1000 and outside_nonzero == 0 and diagonal > 1000
