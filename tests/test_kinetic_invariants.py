@@ -8,7 +8,7 @@ these tests):
   symmetric in T, with P_xy < 0 everywhere and wall slip (test_kinetic_cpu.cpp:177-199);
 - the converged solution does not depend on the sweep order, the inner sweeps, the wall update or
   the block passes (tests/kinetic/test_kinetic_phase5.cpp:200-239).
-The meshes are this build's box generator (same sizes and patches; the reference's distorted-box
+The meshes come from gsis-sim_amd/meshgen.py's box generator (same sizes and patches; the reference's distorted-box
 node jitter uses its own random numbers, so the third mesh differs in its interior nodes)."""
 
 from __future__ import annotations
