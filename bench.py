@@ -180,16 +180,18 @@ def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8, stre
 
 def time_to_mdot(g, mg, args):
     """Metric 2: wall time until the inlet Mdot is within 1e-4 relative of the converged FP64 Mdot,
-    for the FP64, FP32 and mixed forms, on the 40k-cell tube (24^3 velocities, tanh-stretched unless
-    --uniform-grid, delta_rp = 10)."""
+    for the FP64, FP32 and mixed forms, on the 40k-cell tube with 24^3 velocities or (--ttm-mesh 135k)
+    the 135k-cell tube with 32^3, tanh-stretched unless --uniform-grid, delta_rp = --ttm-delta."""
     import gsis_sim_amd.cis_run as cr
 
-    tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
+    big = args.ttm_mesh == "135k"
+    tube_args = WORKLOADS["tube135k_32cubed_fp32" if big else "tube40k_24cubed_fp64"][0]
+    n_vel, delta = (32 if big else 24), args.ttm_delta
     runs = {}
     with tempfile.TemporaryDirectory() as d:
         for prec, kin in (("fp64", {"precision": "fp64"}), ("fp32", {"precision": "fp32"}),
                           ("mixed", {"precision": "fp32", "moment_sums": "mfma"})):
-            path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=10.0, kinetic=kin, name=f"tube_{prec}",
+            path = mg.tube_case(d, n_vel=n_vel, vrange=6.0, delta=delta, kinetic=kin, name=f"tube_{prec}",
                                 stretch=0.0 if args.uniform_grid else 1.5, **tube_args)
             setup = g.Setup(path)
             sv = g.Solver(setup)
@@ -200,7 +202,8 @@ def time_to_mdot(g, mg, args):
             runs[prec] = h
     ref = float(runs["fp64"]["mdot_in"][-1])
     out = {"metric": "time-to-Mdot (1e-4 rel)", "unit": "s",
-           "workload": "tube40k_24cubed" if args.uniform_grid else "tube40k_24cubed_tanh", "mdot_ref_fp64": ref,
+           "workload": f"tube{args.ttm_mesh}_{n_vel}cubed{'' if args.uniform_grid else '_tanh'}"
+                       + ("" if delta == 10.0 else f"_delta{delta:g}"), "mdot_ref_fp64": ref,
            "residual_target": args.residual_target}
     for prec, h in runs.items():
         t = cr.time_to_within(h, ref, 1e-4)
@@ -283,6 +286,9 @@ def main():
                     help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
                          "within 1e-4 of the converged FP64 value")
     ap.add_argument("--max-iterations", type=int, default=800)
+    ap.add_argument("--ttm-mesh", choices=["40k", "135k"], default="40k",
+                    help="time-to-Mdot tube: 40k cells with 24^3 velocities, or 135k cells with 32^3")
+    ap.add_argument("--ttm-delta", type=float, default=10.0, help="time-to-Mdot rarefaction parameter delta_rp")
     ap.add_argument("--sensitivity", action="store_true",
                     help="cost of dMdot/d delta_rp and dMdot/d alpha relative to one converged forward solve "
                          "(40k tube with a Maxwell wall, FP64)")
