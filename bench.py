@@ -85,19 +85,28 @@ def phase_bytes_per_pair(phase, passes):
     return None
 
 
-def measured_traffic(workload, kernel, precision):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for a plain 2 GiB copy),
-    or None when no summary for this kernel exists."""
+def measured_traffic(workload, kernel=None, precision=None):
+    """HBM bytes from the committed rocprofv3 PMC summary of this workload (FETCH_SIZE and WRITE_SIZE
+    runs, each scaled by what the counter reads for a plain 2 GiB copy): per launch of `kernel`, or,
+    with kernel=None, per CIS iteration over every kernel of the iteration. Returns (bytes, source)
+    or (None, None) when no summary exists."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     data = json.load(open(path))
+    source = f"profiles/pmc_traffic_{workload}.json" + (f" (made at commit {data['commit']})" if "commit" in data else "")
+    kernels = data.get("kernels", {})
+    if kernel is None:
+        # initialisation kernels run once before the profiled iterations
+        step = {k: v for k, v in kernels.items() if not k.startswith(("k_set_equilibrium", "k_from_double"))}
+        total = sum((k["fetch_bytes"] + k["write_bytes"]) * k["launches"] for k in step.values())
+        return int(total / data.get("steps", 1)), source
     # every instantiation of the kernel (the sweeps have one per block pass kind), weighted by launches
     prefix = f"k_{kernel}<{'double' if precision == 'fp64' else 'float'}"
-    ks = [k for name, k in data.get("kernels", {}).items() if name == prefix + ">" or name.startswith(prefix + ",")]
+    ks = [k for name, k in kernels.items() if name == prefix + ">" or name.startswith(prefix + ",")]
     n = sum(k["launches"] for k in ks)
-    return None if n == 0 else int(sum((k["fetch_bytes"] + k["write_bytes"]) * k["launches"] for k in ks) / n)
+    return (None, None) if n == 0 else (int(sum((k["fetch_bytes"] + k["write_bytes"]) * k["launches"] for k in ks) / n),
+                                        source)
 
 
 def summarize_trace(stats_csv, workload, command):
@@ -114,11 +123,14 @@ def summarize_trace(stats_csv, workload, command):
     return "\n".join(lines) + "\n"
 
 
-def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload, commands):
+def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload, commands, steps=1):
     """HBM bytes per launch of each kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
-    dispatch), each scaled by the reading of the same counter for a plain 2 GiB device copy."""
+    dispatch), each scaled by the reading of the same counter for a plain 2 GiB device copy. `steps`:
+    the CIS iterations the profiled command ran (warmup included); the source commit is recorded when
+    the summary is written inside a git checkout."""
     import collections
     import csv
+    import subprocess
 
     def load(path):
         d = collections.defaultdict(list)
@@ -132,7 +144,12 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload
     copy = "__amd_rocclr_copyBuffer"
     sf = copy_bytes / (sum(cf[copy]) / len(cf[copy]) * 1024)
     sw = copy_bytes / (sum(cw[copy]) / len(cw[copy]) * 1024)
-    out = {"workload": workload, "fetch_scale": sf, "write_scale": sw,
+    try:
+        commit = subprocess.run(["git", "-C", ROOT, "rev-parse", "--short", "HEAD"], capture_output=True, text=True,
+                                check=True).stdout.strip()
+    except (OSError, subprocess.CalledProcessError):
+        commit = None
+    out = {"workload": workload, "steps": steps, "commit": commit, "fetch_scale": sf, "write_scale": sw,
            "note": "bytes per launch = counter KiB * 1024 * scale; scale = 2 GiB / counter reading for a plain "
                    "2 GiB device copy", "commands": commands, "kernels": {}}
     for k in sorted(f):
@@ -148,34 +165,36 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, case_dir, tube_args, precision, iterations=10, n_vel=8, stretch=0.0):
-    """Oracle (CPU restatement, OpenMP) on the same tube mesh with an 8^3 velocity grid of the same
-    kind (about 10-20 s of CPU work)."""
+def cpu_baseline(g, setup, iterations=1):
+    """The oracle (CPU restatement of the reference's CPU backend, OpenMP over the host cores this
+    process may use: OMP_NUM_THREADS, else all) on the same workload as the GPU line: the same case,
+    mesh, velocity grid and precision. One untimed CIS iteration, then `iterations` timed ones (the
+    reference's timing leaves out the first iteration)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    import gsis_sim_amd.meshgen as mg
 
-    threads = oracle.set_threads(min(16, os.cpu_count() or 1))
-    path = mg.tube_case(case_dir, n_vel=n_vel, vrange=6.0, delta=10.0, kinetic={"precision": precision}, name="cpu_tube",
-                        stretch=stretch, **tube_args)
-    s = g.Setup(path)
-    cb = oracle.CpuBackend(s, "oracle", ops_type=g.BackendOps)
-    sv = g.Solver(s, cb.ops)
+    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
+    cb = oracle.CpuBackend(setup, "oracle", ops_type=g.BackendOps)
+    sv = g.Solver(setup, cb.ops)
     sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    t0 = time.perf_counter()
     sv.prepare()
-    sv.sweep()  # first iteration excluded, as the reference's timing does
+    sv.sweep()
+    t_first = time.perf_counter() - t0
+    log(f"cpu baseline: untimed first iteration {t_first:.1f} s")
     t0 = time.perf_counter()
     for _ in range(iterations):
         sv.prepare()
         sv.sweep()
     dt = time.perf_counter() - t0
-    inner = 4
-    updates = s.n_cells * s.nv * inner * iterations
+    inner = int(setup.describe()["params"]["inner_sweeps"])
+    updates = setup.n_cells * setup.nv * inner * iterations
     sv.close()
+    cb.close()
     return dict(value=updates / dt / 1e9, unit="Gcell·vel updates/s", cores=threads, kind="port",
-                sample=f"CPU oracle (OpenMP, {threads} threads), same {s.n_cells}-cell tube with a {n_vel}^3 "
-                       f"{'tanh-stretched' if stretch > 0 else 'uniform'} velocity grid, "
-                       f"{iterations} CIS iterations after 1 untimed ({dt:.1f} s)")
+                sample=f"CPU oracle (OpenMP, {threads} threads) on the same workload ({setup.n_cells} cells x "
+                       f"{setup.nv} velocities, same case and precision): {iterations} timed CIS iteration(s) "
+                       f"after 1 untimed ({dt:.1f} s timed, {t_first:.1f} s untimed)")
 
 
 def time_to_mdot(g, mg, args):
@@ -255,7 +274,8 @@ def sensitivity_bench(g, mg, args):
                "sensitivity_seconds": prob.stats["sensitivity_seconds"],
                "cost_ratio": prob.stats["sensitivity_seconds"] / fwd_s,
                "dmdot_ddelta": float(d_t.grad), "dmdot_dalpha": float(a_t.grad),
-               "krylov": {k: v for k, v in prob.stats.items() if "krylov" in k}}
+               "krylov": {k: v for k, v in prob.stats.items() if k.endswith(("_iterations", "_relative_residual"))
+                          and not k.startswith("forward")}}
         log(f"sensitivities {out['dmdot_ddelta']:.8e} {out['dmdot_dalpha']:.8e} in {out['sensitivity_seconds']:.1f} s")
         if args.fd_check:
             def mdot_at(dl, al):
@@ -282,6 +302,7 @@ def main():
     ap.add_argument("--workload", default="tube40k_24cubed_tanh_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--uniform-grid", action="store_true", help="time-to-Mdot on the reference's uniform velocity grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iterations", type=int, default=1, help="timed CIS iterations of the CPU baseline")
     ap.add_argument("--time-to-mdot", action="store_true",
                     help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
                          "within 1e-4 of the converged FP64 value")
@@ -312,13 +333,14 @@ def main():
                     help="write per-kernel HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs "
                          "(no GPU work)")
     ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
+    ap.add_argument("--pmc-steps", type=int, default=1, help="CIS iterations of the profiled command (--summarize-pmc)")
     args = ap.parse_args()
     if args.summarize_trace:
         md = summarize_trace(args.summarize_trace[0], args.workload, "; ".join(args.command))
         open(args.summarize_trace[1], "w").write(md)
         return
     if args.summarize_pmc:
-        out = summarize_pmc(*args.summarize_pmc[:4], args.workload, args.command)
+        out = summarize_pmc(*args.summarize_pmc[:4], args.workload, args.command, steps=args.pmc_steps)
         json.dump(out, open(args.summarize_pmc[4], "w"), indent=1)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -364,7 +386,7 @@ def main():
         memory = {"backend_bytes": int(be.memory_bytes()), "device_used_bytes": int(total_b - free_b),
                   "device_total_bytes": int(total_b)}
         passes = setup.view().block_passes
-        inner = 4
+        inner = int(setup.describe()["params"]["inner_sweeps"])
         for i in range(args.warmup):
             sv.prepare()
             info = sv.sweep()
@@ -389,38 +411,49 @@ def main():
         pairs = setup.n_cells * setup.nv
         updates = pairs * inner * args.steps
         value = updates / elapsed / 1e9
-        # launches per step of each phase (HIP events bracket each launch group on the backend stream)
+        step_s = elapsed / args.steps
+        # Roofline of the kernel work of one CIS iteration, in the algorithmic bytes the kinetic step
+        # must move: per update (one pair through one inner sweep) f read + f written + one
+        # intermediate written and read per block pass, (2 + 2 K) s; plus s per pair for the moment pass
+        b_alg = (2 + 2 * passes) * s_bytes
+        bytes_step = pairs * (inner * b_alg + s_bytes)
+        achieved = bytes_step / step_s / 1e9
+        traffic, traffic_source = measured_traffic(args.workload)
+        # the dominant kernel, from HIP events on the backend's own stream (a diagnostic: its bytes
+        # include the chunk workspace the algorithmic figure does not count)
         launches = {"gradient": inner * n_chunks, "residual": inner * n_chunks,
                     "sweep_forward": inner * n_chunks * passes, "sweep_backward": inner * n_chunks * passes}
         kernel_phases = {k: phase.get(k, 0.0) for k in launches}
         dom = max(kernel_phases, key=kernel_phases.get)
-        n_launch = launches[dom] * args.steps
-        avg_launch_s = kernel_phases[dom] / n_launch
-        bytes_per_launch = phase_bytes_per_pair(dom, passes) * s_bytes * pairs / n_chunks
-        achieved = bytes_per_launch / avg_launch_s / 1e9
-        b_alg = (2 + 2 * passes) * s_bytes
+        avg_launch_s = kernel_phases[dom] / (launches[dom] * args.steps)
+        dom_bytes = phase_bytes_per_pair(dom, passes) * s_bytes * pairs / n_chunks
+        dom_traffic, dom_source = measured_traffic(args.workload, dom, precision)
         result = {
             "metric": "Gcell·vel updates/s", "value": round(value, 4), "unit": "Gcell·vel updates/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * step_s, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if s_bytes == 8 else "f32",
             "data": "synthetic",
             "config": {"workload": args.workload, "cells": setup.n_cells, "velocities": setup.nv,
                        "inner_sweeps": inner, "block_passes": passes, "sweep_blocks": setup.n_blocks,
                        "chunk_width": chunk, "delta_rp": delta,
                        "velocity_grid": f"tanh-stretched (stretch {stretch}) on [-6, 6]^3" if stretch > 0 else "uniform on [-6, 6]^3"},
-            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "traffic": measured_traffic(args.workload, dom, precision),
-                         "avg_launch_ms": round(1e3 * avg_launch_s, 4),
-                         "bytes_per_launch": int(bytes_per_launch)},
-            "step_roofline": {"bytes_per_update": b_alg, "achieved": round(value * b_alg, 2),
-                              "frac": round(value * b_alg / HBM_PEAK_GBS, 4)},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                         "work": "one CIS iteration: inner_sweeps updates per (cell, velocity) pair + one moment pass",
+                         "bytes_per_update": b_alg, "algorithmic_bytes_per_iteration": int(bytes_step),
+                         "traffic_unit": "HBM bytes per CIS iteration, all kernels (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+                         "traffic_source": traffic_source},
+            "dominant_kernel": {"kernel": dom, "avg_launch_ms": round(1e3 * avg_launch_s, 4),
+                                "alg_bytes_per_launch": int(dom_bytes),
+                                "achieved_gbs": round(dom_bytes / avg_launch_s / 1e9, 2),
+                                "frac": round(dom_bytes / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                                "pmc_bytes_per_launch": dom_traffic, "pmc_source": dom_source},
             "phase_seconds_per_step": {k: round(v / args.steps, 5) for k, v in phase.items()},
             "memory": memory,
         }
         if not args.no_cpu_baseline:
             log("cpu baseline")
-            result["cpu_baseline"] = cpu_baseline(g, d, tube_args, precision, stretch=stretch)
+            result["cpu_baseline"] = cpu_baseline(g, setup, iterations=args.cpu_iterations)
         else:
             result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

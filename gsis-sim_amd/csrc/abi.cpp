@@ -388,6 +388,8 @@ int gsk_backend_gsis_correction(gsk_backend* b, const double* w_half, const doub
                                 int32_t* replaced_flags, size_t n_flags, long long* replaced) {
   return guarded([&] {
     need(b, "backend");
+    need(w_half, "w_half");
+    need(w_new, "w_new");
     const long long r = b->b->gsis_correction(w_half, w_new, n_w, replaced_flags, replaced_flags ? n_flags : 0);
     if (replaced) *replaced = r;
   });
@@ -572,6 +574,8 @@ int gsk_solver_apply_gsis_correction(gsk_solver* sv, const double* w_half, const
                                      long long* replaced) {
   return guarded([&] {
     need(sv, "solver");
+    need(w_half, "w_half");
+    need(w_new, "w_new");
     const long long r = sv->sv->apply_gsis_correction(w_half, w_new, n);
     if (replaced) *replaced = r;
   });

@@ -55,7 +55,10 @@ class SteadyProblem:
         self.delta_rp = delta_rp if delta_rp is not None else 1.0 / (math.sqrt(2.0) * solver.mu0())
         self.alpha = alpha
         self.eps = eps
-        self.boundary_iterations = 1  # the case's [kinetic].boundary_iterations, restored after a solve
+        # the case's own settings, restored after a sensitivity solve changes them
+        params = solver.setup.describe()["params"]
+        self.boundary_iterations = int(params["boundary_iterations"])
+        self.newton_tolerances = (solver.setup.view().deq_tolerance, 1e-10)
         self.log = None  # optional progress callback (message) for long solves
         self.f0 = None
         self.R0 = None
@@ -67,7 +70,7 @@ class SteadyProblem:
         if delta_rp is not None:
             self.delta_rp = float(delta_rp)
             self.sv.set_mu0(1.0 / (math.sqrt(2.0) * self.delta_rp))
-        if alpha is not None:
+        if alpha is not None and self.wall_patch is not None:
             self.alpha = float(alpha)
             self.sv.set_accommodation(self.wall_patch, self.alpha)
 
@@ -261,38 +264,43 @@ class SteadyProblem:
                       max_iterations: int = 400, method: str = "bicgstab", polish: int = 0) -> dict:
         """dMdot/dp at the converged state for each named parameter."""
         t0 = time.perf_counter()
+        if "alpha" in params and self.wall_patch is None:
+            raise ValueError("dMdot/dalpha needs the Maxwell-wall patch (wall_patch)")
         self.sv.set_newton_tolerances(0.0, 0.0)
         self.sv.set_boundary_iterations(10)
-        self._capture()
         out = {}
-        for p in params:
-            dR, dM = self._param_derivative(p)
-            b = -dR
-            bn = float(torch.linalg.vector_norm(b))
-            if method == "gmres":
-                s, its, rel = self._gmres(b, tol, restart, max_iterations)
-            else:
-                s, its, rel = self._bicgstab(b, tol, max_iterations)
-            # BiCGStab's recurrence residual drifts from the true one b - J s on an ill-conditioned
-            # system: polish with restarted GMRES on the true residual (its residual cannot grow)
-            r_true = b - self._jv(s)[0]
-            true_rel = float(torch.linalg.vector_norm(r_true)) / bn
-            if method != "gmres" and polish > 0 and true_rel > tol:
-                d, k, _ = self._gmres(r_true, tol * bn / float(torch.linalg.vector_norm(r_true)), restart, polish)
-                s = s + d
-                its += k
-                true_rel = float(torch.linalg.vector_norm(b - self._jv(s)[0])) / bn
-            if self.log is not None:
-                self.log(f"{p}: {its} Krylov iterations, true relative residual {true_rel:.2e}")
-            _, dMs = self._jv(s)
-            out[p] = dM + dMs
-            self.stats[f"{p}_krylov_iterations"] = its
-            self.stats[f"{p}_krylov_relative_residual"] = rel
-            self.stats[f"{p}_true_relative_residual"] = true_rel
-        self.sv.set_newton_tolerances(self.sv.setup.view().deq_tolerance, 1e-10)
-        self.sv.set_boundary_iterations(self.boundary_iterations)
-        self._set_f(self.f0)  # leave the solver at f*
-        self.sv.prepare()
+        try:
+            self._capture()
+            for p in params:
+                dR, dM = self._param_derivative(p)
+                b = -dR
+                bn = float(torch.linalg.vector_norm(b))
+                if method == "gmres":
+                    s, its, rel = self._gmres(b, tol, restart, max_iterations)
+                else:
+                    s, its, rel = self._bicgstab(b, tol, max_iterations)
+                # BiCGStab's recurrence residual drifts from the true one b - J s on an ill-conditioned
+                # system: polish with restarted GMRES on the true residual (its residual cannot grow)
+                r_true = b - self._jv(s)[0]
+                true_rel = float(torch.linalg.vector_norm(r_true)) / bn
+                if method != "gmres" and polish > 0 and true_rel > tol:
+                    d, k, _ = self._gmres(r_true, tol * bn / float(torch.linalg.vector_norm(r_true)), restart, polish)
+                    s = s + d
+                    its += k
+                    true_rel = float(torch.linalg.vector_norm(b - self._jv(s)[0])) / bn
+                if self.log is not None:
+                    self.log(f"{p}: {its} Krylov iterations, true relative residual {true_rel:.2e}")
+                _, dMs = self._jv(s)
+                out[p] = dM + dMs
+                self.stats[f"{p}_krylov_iterations"] = its
+                self.stats[f"{p}_krylov_relative_residual"] = rel
+                self.stats[f"{p}_true_relative_residual"] = true_rel
+        finally:
+            self.sv.set_newton_tolerances(*self.newton_tolerances)
+            self.sv.set_boundary_iterations(self.boundary_iterations)
+            if self.f0 is not None:
+                self._set_f(self.f0)  # leave the solver at f*
+                self.sv.prepare()
         self.stats["sensitivity_seconds"] = time.perf_counter() - t0
         return out
 
@@ -304,17 +312,18 @@ class ConvergedMdot(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):
-        problem.set_parameters(float(delta_rp), float(alpha))
+        problem.set_parameters(float(delta_rp), float(alpha) if problem.wall_patch is not None else None)
         mdot = problem.solve(residual_target=residual_target)
         ctx.problem = problem
         return torch.tensor(mdot, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad):
-        sens = ctx.problem.sensitivities(polish=getattr(ctx.problem, "polish", 0))
+        params = ("delta_rp", "alpha") if ctx.problem.wall_patch is not None else ("delta_rp",)
+        sens = ctx.problem.sensitivities(params=params, polish=getattr(ctx.problem, "polish", 0))
         g = float(grad)
         return (torch.tensor(g * sens["delta_rp"], dtype=torch.float64),
-                torch.tensor(g * sens["alpha"], dtype=torch.float64), None, None)
+                torch.tensor(g * sens.get("alpha", 0.0), dtype=torch.float64), None, None)
 
 
 def converged_mdot(delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem,
