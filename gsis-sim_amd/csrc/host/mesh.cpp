@@ -441,204 +441,7 @@ CellAdjacency build_cell_adjacency(const PolyMesh& m) {
   return a;
 }
 
-namespace {
-std::vector<std::vector<Index>> bfs_levels(const CellAdjacency& adj, Index start, std::vector<long long>& seen,
-                                           long long stamp) {
-  std::vector<std::vector<Index>> levels{{start}};
-  seen[static_cast<std::size_t>(start)] = stamp;
-  while (true) {
-    std::vector<Index> next;
-    for (const Index c : levels.back()) {
-      for (Offset k = adj.offsets[static_cast<std::size_t>(c)]; k < adj.offsets[static_cast<std::size_t>(c) + 1]; ++k) {
-        const Index nb = adj.cells[static_cast<std::size_t>(k)];
-        if (seen[static_cast<std::size_t>(nb)] != stamp) {
-          seen[static_cast<std::size_t>(nb)] = stamp;
-          next.push_back(nb);
-        }
-      }
-    }
-    if (next.empty()) break;
-    levels.push_back(std::move(next));
-  }
-  return levels;
-}
-}  // namespace
-
-std::vector<Index> rcm_ordering(const PolyMesh& m) {
-  const CellAdjacency adj = build_cell_adjacency(m);
-  const Index n = m.n_cells;
-  std::vector<Index> order;
-  order.reserve(static_cast<std::size_t>(n));
-  std::vector<char> placed(static_cast<std::size_t>(n), 0);
-  std::vector<long long> seen(static_cast<std::size_t>(n), -1);
-  long long stamp = 0;
-  std::vector<Index> by_deg(static_cast<std::size_t>(n));
-  std::iota(by_deg.begin(), by_deg.end(), 0);
-  std::stable_sort(by_deg.begin(), by_deg.end(), [&](Index a, Index b) { return adj.degree(a) < adj.degree(b); });
-  auto deg_less = [&](Index a, Index b) {
-    return adj.degree(a) != adj.degree(b) ? adj.degree(a) < adj.degree(b) : a < b;
-  };
-  for (const Index seed : by_deg) {
-    if (placed[static_cast<std::size_t>(seed)]) continue;
-    Index start = seed;
-    auto levels = bfs_levels(adj, start, seen, ++stamp);
-    for (int it = 0; it < 16; ++it) {
-      const std::vector<Index>& last = levels.back();
-      Index cand = last.front();
-      for (const Index c : last)
-        if (deg_less(c, cand)) cand = c;
-      auto cl = bfs_levels(adj, cand, seen, ++stamp);
-      if (cl.size() <= levels.size()) break;
-      start = cand;
-      levels = std::move(cl);
-    }
-    const std::size_t first = order.size();
-    order.push_back(start);
-    placed[static_cast<std::size_t>(start)] = 1;
-    std::vector<Index> nbrs;
-    for (std::size_t head = first; head < order.size(); ++head) {
-      const Index c = order[head];
-      nbrs.clear();
-      for (Offset k = adj.offsets[static_cast<std::size_t>(c)]; k < adj.offsets[static_cast<std::size_t>(c) + 1]; ++k) {
-        const Index nb = adj.cells[static_cast<std::size_t>(k)];
-        if (!placed[static_cast<std::size_t>(nb)]) {
-          placed[static_cast<std::size_t>(nb)] = 1;
-          nbrs.push_back(nb);
-        }
-      }
-      std::sort(nbrs.begin(), nbrs.end(), deg_less);
-      order.insert(order.end(), nbrs.begin(), nbrs.end());
-    }
-  }
-  std::reverse(order.begin(), order.end());
-  return order;
-}
-
-namespace {
-
-// Recursive coordinate bisection along the longest bounding-box axis; ties by (coordinate, index).
-void rcb_split(std::vector<Index>& cells, int P, const std::vector<Vec3>& ctr, std::vector<std::vector<Index>>& out) {
-  if (P <= 1 || cells.size() <= 1) {
-    out.push_back(cells);
-    return;
-  }
-  Vec3 lo = ctr[static_cast<std::size_t>(cells.front())], hi = lo;
-  for (const Index c : cells) {
-    const Vec3& x = ctr[static_cast<std::size_t>(c)];
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = std::min(lo[a], x[a]);
-      hi[a] = std::max(hi[a], x[a]);
-    }
-  }
-  int axis = 0;
-  for (int a = 1; a < 3; ++a)
-    if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
-  const int P1 = P / 2, P2 = P - P1;
-  const auto n1 = static_cast<std::ptrdiff_t>(
-      std::llround(static_cast<double>(cells.size()) * static_cast<double>(P1) / static_cast<double>(P)));
-  auto less = [&](Index a, Index b) {
-    const double xa = ctr[static_cast<std::size_t>(a)][axis], xb = ctr[static_cast<std::size_t>(b)][axis];
-    return xa != xb ? xa < xb : a < b;
-  };
-  std::nth_element(cells.begin(), cells.begin() + n1, cells.end(), less);
-  std::vector<Index> left(cells.begin(), cells.begin() + n1), right(cells.begin() + n1, cells.end());
-  std::sort(left.begin(), left.end());
-  std::sort(right.begin(), right.end());
-  rcb_split(left, P1, ctr, out);
-  rcb_split(right, P2, ctr, out);
-}
-
-// Reverse Cuthill-McKee restricted to one block (adjacency filtered to the block).
-std::vector<Index> block_rcm(const std::vector<Index>& cells, const CellAdjacency& adj, const std::vector<int>& block_of,
-                             int b, std::vector<char>& placed, std::vector<long long>& mark, long long& stamp) {
-  auto in_block = [&](Index c) { return block_of[static_cast<std::size_t>(c)] == b; };
-  auto degree = [&](Index c) {
-    int d = 0;
-    for (Offset k = adj.offsets[static_cast<std::size_t>(c)]; k < adj.offsets[static_cast<std::size_t>(c) + 1]; ++k)
-      if (in_block(adj.cells[static_cast<std::size_t>(k)])) ++d;
-    return d;
-  };
-  auto by_degree = [&](Index x, Index y) {
-    const int dx = degree(x), dy = degree(y);
-    return dx != dy ? dx < dy : x < y;
-  };
-  auto last_level = [&](Index start) {
-    ++stamp;
-    std::vector<Index> level{start};
-    mark[static_cast<std::size_t>(start)] = stamp;
-    std::vector<Index> last = level;
-    while (!level.empty()) {
-      last = level;
-      std::vector<Index> next;
-      for (const Index c : level) {
-        for (Offset k = adj.offsets[static_cast<std::size_t>(c)]; k < adj.offsets[static_cast<std::size_t>(c) + 1]; ++k) {
-          const Index nb = adj.cells[static_cast<std::size_t>(k)];
-          if (in_block(nb) && mark[static_cast<std::size_t>(nb)] != stamp) {
-            mark[static_cast<std::size_t>(nb)] = stamp;
-            next.push_back(nb);
-          }
-        }
-      }
-      level = std::move(next);
-    }
-    return last;
-  };
-  std::vector<Index> seeds = cells;
-  std::sort(seeds.begin(), seeds.end(), by_degree);
-  std::vector<Index> order;
-  order.reserve(cells.size());
-  std::vector<Index> nbrs;
-  for (const Index seed : seeds) {
-    if (placed[static_cast<std::size_t>(seed)]) continue;
-    const std::vector<Index> last = last_level(seed);
-    const Index start = *std::min_element(last.begin(), last.end(), by_degree);
-    const std::size_t first = order.size();
-    order.push_back(start);
-    placed[static_cast<std::size_t>(start)] = 1;
-    for (std::size_t head = first; head < order.size(); ++head) {
-      const Index c = order[head];
-      nbrs.clear();
-      for (Offset k = adj.offsets[static_cast<std::size_t>(c)]; k < adj.offsets[static_cast<std::size_t>(c) + 1]; ++k) {
-        const Index nb = adj.cells[static_cast<std::size_t>(k)];
-        if (in_block(nb) && !placed[static_cast<std::size_t>(nb)]) {
-          placed[static_cast<std::size_t>(nb)] = 1;
-          nbrs.push_back(nb);
-        }
-      }
-      std::sort(nbrs.begin(), nbrs.end(), by_degree);
-      order.insert(order.end(), nbrs.begin(), nbrs.end());
-    }
-  }
-  std::reverse(order.begin(), order.end());
-  return order;
-}
-
-}  // namespace
-
-std::vector<Index> rcb_ordering(const PolyMesh& m, const MeshGeometry& g, int P, std::vector<Index>& block_offsets) {
-  const auto n = static_cast<std::size_t>(m.n_cells);
-  std::vector<Index> all(n);
-  std::iota(all.begin(), all.end(), 0);
-  std::vector<std::vector<Index>> groups;
-  rcb_split(all, P, g.cell_center, groups);
-  std::vector<int> block_of(n, -1);
-  for (std::size_t b = 0; b < groups.size(); ++b)
-    for (const Index c : groups[b]) block_of[static_cast<std::size_t>(c)] = static_cast<int>(b);
-  const CellAdjacency adj = build_cell_adjacency(m);
-  std::vector<char> placed(n, 0);
-  std::vector<long long> mark(n, -1);
-  long long stamp = 0;
-  std::vector<Index> new_to_old;
-  new_to_old.reserve(n);
-  block_offsets.assign(1, 0);
-  for (std::size_t b = 0; b < groups.size(); ++b) {
-    const std::vector<Index> ord = block_rcm(groups[b], adj, block_of, static_cast<int>(b), placed, mark, stamp);
-    GSK_CHECK(ord.size() == groups[b].size(), "internal error: block ordering lost cells");
-    new_to_old.insert(new_to_old.end(), ord.begin(), ord.end());
-    block_offsets.push_back(static_cast<Index>(new_to_old.size()));
-  }
-  return new_to_old;
-}
+// rcm_ordering and rcb_ordering: ordering.cpp
 
 std::vector<Index> invert_permutation(const std::vector<Index>& perm) {
   std::vector<Index> inv(perm.size(), kInvalid);

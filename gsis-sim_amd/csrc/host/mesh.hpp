@@ -7,7 +7,9 @@
 //   face / cell geometry                src/mesh/geometry.cpp:10-125
 //   cell->face CSR (ascending face)     src/mesh/fv_geometry.cpp:53-91
 //   LSQ coefficients c_k = w G^-1 r_k   src/mesh/fv_geometry.cpp:95-176
-//   adjacency / RCM / renumbering       src/mesh/ordering.cpp:9-236
+//   adjacency / renumbering             src/mesh/ordering.cpp:9-29, :122-236
+//   RCM and RCB block orderings         src/mesh/ordering.cpp:30-120, src/kinetic/kinetic_setup.cpp:46-178
+//                                       (ordering.cpp here)
 //   contiguous partition                src/mesh/ordering.cpp:238-250
 
 #include <array>
