@@ -114,6 +114,7 @@ EXPORTS = [
     "gsk_backend_scale_distribution", "gsk_backend_update_cell_params", "gsk_backend_set_inflow",
     "gsk_backend_boundary_sums", "gsk_backend_sweep", "gsk_backend_gsis_correction",
     "gsk_backend_internal_face_mass_flux", "gsk_backend_take_timings", "gsk_backend_deq_counts",
+    "gsk_backend_gsis_correction_noalloc",
     "gsk_solver_create", "gsk_solver_create_with_backend", "gsk_solver_destroy", "gsk_solver_initialize_uniform",
     "gsk_solver_initialize_equilibrium", "gsk_solver_prepare", "gsk_solver_update_mass_flow_pressure",
     "gsk_solver_sweep", "gsk_solver_patch_mass_flow", "gsk_solver_moments", "gsk_solver_boundary_fluxes",
@@ -123,6 +124,10 @@ EXPORTS = [
     "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
     "gsk_solver_set_boundary_iterations",
 ]
+
+# fixed device scratch of gsk_backend_gsis_correction_noalloc (GSK_NOALLOC_SCRATCH_BYTES, GSK_NOALLOC_MAX_CELLS)
+NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024
+NOALLOC_MAX_CELLS = (NOALLOC_SCRATCH_BYTES - 256) // 80
 
 _lib_handle = None
 
@@ -306,6 +311,17 @@ class Backend:
         rep = C.c_longlong()
         _check(lib().gsk_backend_gsis_correction(self._h, _dptr(wh), _dptr(wn), C.c_size_t(wh.size), _iptr(flags),
                                                  C.c_size_t(flags.size), C.byref(rep)))
+        return rep.value, flags
+
+    def gsis_correction_noalloc(self, w_half: np.ndarray, w_new: np.ndarray):
+        """gsis_correction through the entry point that allocates no device memory (fixed per-device
+        scratch; at most NOALLOC_MAX_CELLS cells)."""
+        wh = np.ascontiguousarray(w_half, np.float64).ravel()
+        wn = np.ascontiguousarray(w_new, np.float64).ravel()
+        flags = np.zeros(self.setup.n_cells, np.int32)
+        rep = C.c_longlong()
+        _check(lib().gsk_backend_gsis_correction_noalloc(self._h, _dptr(wh), _dptr(wn), C.c_size_t(wh.size),
+                                                         _iptr(flags), C.c_size_t(flags.size), C.byref(rep)))
         return rep.value, flags
 
     def internal_face_mass_flux(self, cells: np.ndarray, entries: np.ndarray) -> np.ndarray:

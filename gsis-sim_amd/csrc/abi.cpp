@@ -395,6 +395,18 @@ int gsk_backend_gsis_correction(gsk_backend* b, const double* w_half, const doub
   });
 }
 
+int gsk_backend_gsis_correction_noalloc(gsk_backend* b, const double* w_half, const double* w_new, size_t n_w,
+                                        int32_t* replaced_flags, size_t n_flags, long long* replaced) {
+  return guarded([&] {
+    need(b, "backend");
+    need(w_half, "w_half");
+    need(w_new, "w_new");
+    const long long r =
+        b->b->gsis_correction_noalloc(w_half, w_new, n_w, replaced_flags, replaced_flags ? n_flags : 0);
+    if (replaced) *replaced = r;
+  });
+}
+
 int gsk_backend_internal_face_mass_flux(gsk_backend* b, const int32_t* cells, const int32_t* entries, size_t n,
                                         double* out) {
   GSK_BACKEND_CALL(b, internal_face_mass_flux(cells, entries, n, out));

@@ -32,6 +32,17 @@ class Backend {
   virtual long long gsis_correction(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
                                     std::size_t n_flags) = 0;
   virtual void internal_face_mass_flux(const int* cells, const int* entries, std::size_t n, double* out) = 0;
+  // gsis_correction without any device allocation (no reference counterpart; for callers that cannot
+  // allocate): the HIP backend stages w_half / w_new in a fixed per-device scratch array
+  virtual long long gsis_correction_noalloc(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
+                                            std::size_t n_flags) {
+    (void)w_half;
+    (void)w_new;
+    (void)n_w;
+    (void)flags;
+    (void)n_flags;
+    throw std::runtime_error(std::string("backend \"") + name() + "\" has no non-allocating GSIS correction");
+  }
   virtual gsk_timings take_timings() = 0;
   // boundary sums of at least the listed faces, written at their rows of out (20 x n_bfaces); the
   // other rows hold unspecified values. Default: all faces.

@@ -22,9 +22,17 @@
 #include <type_traits>
 #include <vector>
 
+#include <mutex>
+
 #include "../host/backend.hpp"
 #include "../host/core.hpp"
 #include "kinetic_view.hpp"
+
+// Fixed device scratch of the entry points that allocate nothing (Backend::gsis_correction_noalloc):
+// each GPU the library runs on holds its own copy until the process ends. Host code never passes
+// this name to a kernel or a copy (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returns for the current device (noalloc_scratch() below).
+static __device__ __align__(256) unsigned char scratch[GSK_NOALLOC_SCRATCH_BYTES];
 
 namespace gsk {
 namespace dev {
@@ -1169,6 +1177,21 @@ __global__ __launch_bounds__(kBlock) void k_gsis_correction(View<F> v, const dou
   }
 }
 
+// number of nonzero flags, one workgroup, fixed order (the replaced-cell count of a GSIS correction)
+__global__ __launch_bounds__(kBlock) void k_count_nonzero(const int* __restrict__ flags, int n,
+                                                          long long* __restrict__ out) {
+  __shared__ long long lds[kBlock];
+  long long c = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) c += flags[i] != 0 ? 1 : 0;
+  lds[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (static_cast<int>(threadIdx.x) < h) lds[threadIdx.x] += lds[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+
 template <class F>
 __global__ void k_from_double(const double* __restrict__ src, F* __restrict__ dst, i64 n) {
   for (i64 i = blockIdx.x * static_cast<i64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<i64>(gridDim.x) * blockDim.x)
@@ -1191,6 +1214,25 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
   if (e != hipSuccess) fail_at(file, line, strf("HIP error in %s: %s", what, hipGetErrorString(e)));
 }
 #define HIPCHK(x) hip_check((x), #x, __FILE__, __LINE__)
+
+// Device address of the fixed scratch array on the current device, looked up once per device
+// (each GPU holds its own copy of the array at its own address).
+void* noalloc_scratch() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  GSK_CHECK(dev >= 0 && dev < kMaxDevices, "device ordinal out of range for the fixed scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (ptrs[dev] == nullptr) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GSK_CHECK(p != nullptr, "the fixed scratch array has no device address");
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
 
 // owning device buffer
 template <class T>
@@ -1231,6 +1273,7 @@ class HipBackend final : public Backend {
  public:
   explicit HipBackend(const gsk_setup_view& s) {
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    scratch_ = noalloc_scratch();
     nc_ = s.n_cells;
     nv_ = s.nv;
     nb_ = s.n_bfaces;
@@ -1771,8 +1814,8 @@ class HipBackend final : public Backend {
   }
 
   // no download and no stream synchronisation: the next inner sweep queues behind this one
-  void sweep_unreported(double* scratch, std::size_t n) override {
-    (void)scratch;
+  void sweep_unreported(double* unused, std::size_t n) override {
+    (void)unused;
     GSK_CHECK(n == static_cast<std::size_t>(nc_), "residual output length differs from the cell count");
     launch_sweep();
   }
@@ -1865,6 +1908,35 @@ class HipBackend final : public Backend {
       if (h[c] != 0) ++replaced;
       if (n_flags) flags[c] = h[c];
     }
+    return replaced;
+  }
+
+  // The same correction with w_half / w_new staged in the fixed scratch array instead of buffers
+  // allocated per call; the replaced count is formed on the device. Nothing is allocated and no
+  // timing event is recorded.
+  long long gsis_correction_noalloc(const double* w_half, const double* w_new, std::size_t n_w, int* flags,
+                                    std::size_t n_flags) override {
+    GSK_CHECK(n_w == static_cast<std::size_t>(nc_) * 5, "GSIS correction states length differs from 5 x cells");
+    GSK_CHECK(n_flags == 0 || n_flags == static_cast<std::size_t>(nc_), "flags length differs from the cell count");
+    constexpr std::size_t kTail = 256;  // the count, in the last 256 bytes
+    GSK_CHECK(2 * n_w * sizeof(double) + kTail <= GSK_NOALLOC_SCRATCH_BYTES,
+              strf("%d cells exceed the fixed scratch of the non-allocating GSIS correction (%d bytes: at most %zu "
+                   "cells)",
+                   nc_, GSK_NOALLOC_SCRATCH_BYTES, (GSK_NOALLOC_SCRATCH_BYTES - kTail) / (2 * 5 * sizeof(double))));
+    unsigned char* base = static_cast<unsigned char*>(scratch_);
+    double* wh = reinterpret_cast<double*>(base);
+    double* wn = wh + n_w;
+    long long* count = reinterpret_cast<long long*>(base + GSK_NOALLOC_SCRATCH_BYTES - kTail);
+    HIPCHK(hipMemcpyAsync(wh, w_half, n_w * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(wn, w_new, n_w * sizeof(double), hipMemcpyHostToDevice, stream_));
+    dev::k_gsis_correction<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), wh, wn, negative_mode_, shift_.p, flags_.p);
+    HIPCHK(hipGetLastError());
+    dev::k_count_nonzero<<<1, dev::kBlock, 0, stream_>>>(flags_.p, nc_, count);
+    HIPCHK(hipGetLastError());
+    if (n_flags) HIPCHK(hipMemcpyAsync(flags, flags_.p, n_flags * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    long long replaced = 0;
+    HIPCHK(hipMemcpyAsync(&replaced, count, sizeof(long long), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
     return replaced;
   }
 
@@ -2162,6 +2234,7 @@ class HipBackend final : public Backend {
   }
 
   hipStream_t stream_ = nullptr;
+  void* scratch_ = nullptr;  // this device's fixed scratch array (noalloc_scratch)
   int nc_ = 0, nv_ = 0, nb_ = 0, n_blocks_ = 1, block_passes_ = 1, negative_mode_ = 0;
   int chunk_ = 0, tiles_ = 0, n_sweep_dirs_ = 0, n_mw_ = 0;
   int grid_n_[3] = {0, 0, 0};

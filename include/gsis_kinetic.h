@@ -185,6 +185,17 @@ int gsk_backend_gsis_correction(gsk_backend* b, const double* w_half, const doub
                                 int32_t* replaced_flags, size_t n_flags, long long* replaced);
 int gsk_backend_internal_face_mass_flux(gsk_backend* b, const int32_t* cells, const int32_t* entries, size_t n,
                                         double* out);
+/* gsk_backend_gsis_correction with no device allocation. No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, serving loops, harnesses that
+ * forbid allocation). Same arguments, results and errors as gsk_backend_gsis_correction (the same
+ * kernel), but w_half and w_new are staged in a fixed device array of GSK_NOALLOC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends, so at most GSK_NOALLOC_MAX_CELLS cells
+ * (larger cases return GSK_ERR_INVALID). Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (several backends or threads) race. Records no phase timing. */
+#define GSK_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+#define GSK_NOALLOC_MAX_CELLS ((GSK_NOALLOC_SCRATCH_BYTES - 256) / 80)
+int gsk_backend_gsis_correction_noalloc(gsk_backend* b, const double* w_half, const double* w_new, size_t n_w,
+                                        int32_t* replaced_flags, size_t n_flags, long long* replaced);
 int gsk_backend_take_timings(gsk_backend* b, gsk_timings* out);
 int gsk_backend_deq_counts(gsk_backend* b, gsk_deq_counts* out);
 
