@@ -3,6 +3,7 @@
 #include "setup.hpp"
 
 #include <algorithm>
+#include <map>
 #include <numeric>
 
 namespace gsk {
@@ -62,122 +63,162 @@ double VelocityGrid::max_spacing(int a) const {
   return d;
 }
 
-std::vector<Vec3> make_sweep_directions(const std::string& order) {
-  std::vector<Vec3> d;
-  if (order == "octant" || order == "dir13") {
-    if (order == "dir13") {
-      for (const Vec3& v : {Vec3{1, 0, 0}, Vec3{0, 1, 0}, Vec3{0, 0, 1}, Vec3{1, 1, 0}, Vec3{1, -1, 0}, Vec3{1, 0, 1},
-                            Vec3{1, 0, -1}, Vec3{0, 1, 1}, Vec3{0, 1, -1}})
-        d.push_back(v);
-    }
-    for (const Vec3& v : {Vec3{1, 1, 1}, Vec3{1, 1, -1}, Vec3{1, -1, 1}, Vec3{1, -1, -1}}) d.push_back(v);
-  } else {
-    GSK_CHECK(order == "block", "unknown sweep_order \"" + order + "\"");
+
+// ---------------------------------------------------------------------------------------------
+// The setup is assembled in stages by SetupBuilder; each stage states the rule it applies. The
+// rules are the reference's (src/kinetic/kinetic_setup.cpp:182-599), because the cell numbering,
+// velocity storage order, sweep orders and face tables decide every iterate bit for bit.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// Sweep directions by [kinetic].sweep_order, as unit vectors in this order:
+//   "octant": the four body diagonals (1,1,1) (1,1,-1) (1,-1,1) (1,-1,-1);
+//   "dir13":  the axes x, y, z, the face diagonals (1,1,0) (1,-1,0) (1,0,1) (1,0,-1) (0,1,1) (0,1,-1),
+//             then the four body diagonals;
+//   "block":  none (every velocity sweeps the cells in internal numbering).
+std::vector<Vec3> sweep_direction_table(const std::string& order) {
+  static const int kAxes[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  static const int kFaceDiagonals[6][3] = {{1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+  static const int kBodyDiagonals[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
+  GSK_CHECK(order == "octant" || order == "dir13" || order == "block", "unknown sweep_order \"" + order + "\"");
+  std::vector<Vec3> dirs;
+  auto add = [&dirs](const int(&v)[3]) {
+    const Vec3 x{static_cast<double>(v[0]), static_cast<double>(v[1]), static_cast<double>(v[2])};
+    dirs.push_back(x / norm(x));
+  };
+  if (order == "dir13") {
+    for (const auto& v : kAxes) add(v);
+    for (const auto& v : kFaceDiagonals) add(v);
   }
-  for (Vec3& x : d) x = x / norm(x);
-  return d;
+  if (order != "block")
+    for (const auto& v : kBodyDiagonals) add(v);
+  return dirs;
 }
 
-int sweep_direction_of(const std::vector<Vec3>& dirs, const Vec3& xi) {
-  int best = 0;
-  double best_v = -1.0;
-  for (std::size_t g = 0; g < dirs.size(); ++g) {
-    const double v = std::abs(dot(dirs[g], xi));
-    if (v > best_v + 1e-12) {
-      best_v = v;
-      best = static_cast<int>(g);
+// The sweep direction a velocity follows: the one most nearly parallel or antiparallel to it,
+// largest |d . xi|; a later direction must beat the current best by more than 1e-12 to replace it.
+int nearest_direction(const std::vector<Vec3>& dirs, const Vec3& xi) {
+  int pick = 0;
+  double best = -1.0;
+  for (int g = 0; g < static_cast<int>(dirs.size()); ++g) {
+    const double align = std::abs(dot(dirs[static_cast<std::size_t>(g)], xi));
+    if (align > best + 1e-12) {
+      pick = g;
+      best = align;
     }
   }
-  return best;
+  return pick;
 }
 
-VelocityLayout make_velocity_layout(const VelocityGrid& grid, const std::vector<int>& mirror_axes,
-                                    const std::vector<Vec3>& sweep_dirs) {
+// The axis (0, 1, 2) every face of the patch is normal to, within tol of a unit axis vector; -1 if
+// the faces are not all normal to one axis.
+int patch_normal_axis(const MeshGeometry& g, const Patch& patch, double tol = 1e-8) {
+  int common = -1;
+  for (Index f = patch.start; f < patch.end(); ++f) {
+    const Vec3& n = g.face_normal[static_cast<std::size_t>(f)];
+    int axis = -1;
+    for (int a = 0; a < 3; ++a)
+      if (std::abs(std::abs(n[a]) - 1.0) <= tol) axis = a;
+    if (axis < 0 || (common >= 0 && axis != common)) return -1;
+    common = axis;
+  }
+  return common;
+}
+
+// Axis a face normal lies along within 1e-10 (Maxwell walls reflect through a storage-order xor
+// there), or -1.
+int aligned_axis(const Vec3& n) {
+  int axis = -1;
+  for (int a = 0; a < 3; ++a)
+    if (std::abs(std::abs(n[a]) - 1.0) < 1e-10) axis = a;
+  return axis;
+}
+
+// Storage order of the velocities.
+//  * With mirror axes A (symmetry planes, axis-aligned Maxwell walls): velocities are stored in
+//    mirror orbits. The orbit representatives are the grid points in the lower half of every axis
+//    of A, taken in grid order; the r-th orbit occupies positions r 2^|A| + m, m = 0 .. 2^|A| - 1,
+//    where position m is the representative reflected on the axes A[i] whose bit i is set in m. So
+//    the reflection on A[i] is the storage xor 2^i.
+//  * Without mirrors, with sweep directions: grid points grouped by their sweep direction (groups
+//    in direction order, grid order inside a group), so that a velocity chunk mostly follows one
+//    cell order.
+//  * Otherwise grid order.
+VelocityLayout layout_velocities(const VelocityGrid& grid, std::vector<int> mirror_axes, const std::vector<Vec3>& dirs) {
   VelocityLayout L;
   L.nv = grid.size();
   L.weight = grid.weight();
-  std::vector<int> axes = mirror_axes;
-  std::sort(axes.begin(), axes.end());
-  axes.erase(std::unique(axes.begin(), axes.end()), axes.end());
-  for (const int a : axes) {
+  std::sort(mirror_axes.begin(), mirror_axes.end());
+  mirror_axes.erase(std::unique(mirror_axes.begin(), mirror_axes.end()), mirror_axes.end());
+  for (const int a : mirror_axes) {
     GSK_CHECK(a >= 0 && a < 3, "invalid mirror axis");
     GSK_CHECK(grid.symmetric(a), strf("velocity range of axis %c is not symmetric about 0; mirrors unavailable", "xyz"[a]));
     GSK_CHECK(grid.n(a) % 2 == 0, strf("axis %c of a mirrored velocity grid needs an even point count", "xyz"[a]));
   }
-  const int na = static_cast<int>(axes.size());
-  L.orbit_size = 1 << na;
-  for (int r = 0; r < na; ++r) L.axis_xor[static_cast<std::size_t>(axes[static_cast<std::size_t>(r)])] = 1 << r;
-  L.storage_to_grid.reserve(static_cast<std::size_t>(L.nv));
-  for (int k = 0; k < L.nv; ++k) {
-    const std::array<int, 3> ijk = grid.multi_index(k);
-    bool rep = true;
-    for (const int a : axes)
-      if (ijk[static_cast<std::size_t>(a)] >= grid.n(a) / 2) rep = false;
-    if (!rep) continue;
-    for (int m = 0; m < L.orbit_size; ++m) {
-      std::array<int, 3> mm = ijk;
-      for (int r = 0; r < na; ++r) {
-        if (m & (1 << r)) {
-          const auto a = static_cast<std::size_t>(axes[static_cast<std::size_t>(r)]);
-          mm[a] = grid.n(static_cast<int>(a)) - 1 - mm[a];
-        }
+  const int n_mirror = static_cast<int>(mirror_axes.size());
+  L.orbit_size = 1 << n_mirror;
+  for (int i = 0; i < n_mirror; ++i) L.axis_xor[static_cast<std::size_t>(mirror_axes[static_cast<std::size_t>(i)])] = 1 << i;
+
+  L.storage_to_grid.assign(static_cast<std::size_t>(L.nv), -1);
+  if (n_mirror > 0) {
+    std::size_t next = 0;
+    for (int k = 0; k < L.nv; ++k) {
+      const std::array<int, 3> rep = grid.multi_index(k);
+      bool lower = true;
+      for (const int a : mirror_axes) lower = lower && rep[static_cast<std::size_t>(a)] < grid.n(a) / 2;
+      if (!lower) continue;
+      for (int m = 0; m < L.orbit_size; ++m) {
+        std::array<int, 3> q = rep;
+        for (int i = 0; i < n_mirror; ++i)
+          if ((m >> i) & 1) {
+            const auto a = static_cast<std::size_t>(mirror_axes[static_cast<std::size_t>(i)]);
+            q[a] = grid.n(static_cast<int>(a)) - 1 - q[a];
+          }
+        L.storage_to_grid[next++] = grid.index(q[0], q[1], q[2]);
       }
-      L.storage_to_grid.push_back(grid.index(mm[0], mm[1], mm[2]));
     }
+    GSK_CHECK(next == static_cast<std::size_t>(L.nv), "internal error: velocity orbits do not cover the grid");
+  } else if (!dirs.empty()) {
+    std::vector<std::vector<int>> bucket(dirs.size());
+    for (int k = 0; k < L.nv; ++k) bucket[static_cast<std::size_t>(nearest_direction(dirs, grid.xi(k)))].push_back(k);
+    std::size_t next = 0;
+    for (const auto& b : bucket)
+      for (const int k : b) L.storage_to_grid[next++] = k;
+  } else {
+    std::iota(L.storage_to_grid.begin(), L.storage_to_grid.end(), 0);
   }
-  GSK_CHECK(static_cast<int>(L.storage_to_grid.size()) == L.nv, "internal error: velocity layout size");
-  if (na == 0 && !sweep_dirs.empty()) {
-    std::vector<int> group(static_cast<std::size_t>(L.nv));
-    for (int k = 0; k < L.nv; ++k) group[static_cast<std::size_t>(k)] = sweep_direction_of(sweep_dirs, grid.xi(k));
-    std::stable_sort(L.storage_to_grid.begin(), L.storage_to_grid.end(),
-                     [&](int a, int b) { return group[static_cast<std::size_t>(a)] < group[static_cast<std::size_t>(b)]; });
-  }
-  L.grid_to_storage.assign(static_cast<std::size_t>(L.nv), -1);
-  L.xi_x.resize(static_cast<std::size_t>(L.nv));
-  L.xi_y.resize(static_cast<std::size_t>(L.nv));
-  L.xi_z.resize(static_cast<std::size_t>(L.nv));
-  for (int p = 0; p < L.nv; ++p) {
-    const int k = L.storage_to_grid[static_cast<std::size_t>(p)];
-    L.grid_to_storage[static_cast<std::size_t>(k)] = p;
+
+  const auto nv = static_cast<std::size_t>(L.nv);
+  L.grid_to_storage.assign(nv, -1);
+  L.xi_x.resize(nv);
+  L.xi_y.resize(nv);
+  L.xi_z.resize(nv);
+  for (std::size_t p = 0; p < nv; ++p) {
+    const int k = L.storage_to_grid[p];
+    L.grid_to_storage[static_cast<std::size_t>(k)] = static_cast<int>(p);
     const Vec3 xi = grid.xi(k);
-    L.xi_x[static_cast<std::size_t>(p)] = xi.x;
-    L.xi_y[static_cast<std::size_t>(p)] = xi.y;
-    L.xi_z[static_cast<std::size_t>(p)] = xi.z;
+    L.xi_x[p] = xi.x;
+    L.xi_y[p] = xi.y;
+    L.xi_z[p] = xi.z;
     L.xi_max = std::max(L.xi_max, norm(xi));
   }
   if (grid.stretched()) {
-    L.weight_rel.resize(static_cast<std::size_t>(L.nv));
-    for (int p = 0; p < L.nv; ++p)
-      L.weight_rel[static_cast<std::size_t>(p)] = grid.relative_weight(L.storage_to_grid[static_cast<std::size_t>(p)]);
+    L.weight_rel.resize(nv);
+    for (std::size_t p = 0; p < nv; ++p) L.weight_rel[p] = grid.relative_weight(L.storage_to_grid[p]);
     for (int a = 0; a < 3; ++a)
       for (int i = 0; i < grid.n(a); ++i) L.nodes.push_back(grid.coordinate(a, i));
   }
-  for (const int a : axes)
-    for (int p = 0; p < L.nv; ++p) {
-      const int q = p ^ L.axis_xor[static_cast<std::size_t>(a)];
-      GSK_CHECK(L.storage_to_grid[static_cast<std::size_t>(q)] == grid.mirror(a, L.storage_to_grid[static_cast<std::size_t>(p)]),
-                "internal error: mirror is not a storage xor");
-    }
+  for (int i = 0; i < n_mirror; ++i) {  // the orbit layout's promise, checked
+    const int a = mirror_axes[static_cast<std::size_t>(i)];
+    for (std::size_t p = 0; p < nv; ++p)
+      GSK_CHECK(L.storage_to_grid[p ^ static_cast<std::size_t>(1 << i)] == grid.mirror(a, L.storage_to_grid[p]),
+                "internal error: a mirror is not a storage xor");
+  }
   return L;
 }
 
-int symmetry_patch_axis(const MeshGeometry& g, const Patch& patch, double tol) {
-  int axis = -1;
-  for (Index f = patch.start; f < patch.end(); ++f) {
-    const Vec3& n = g.face_normal[static_cast<std::size_t>(f)];
-    int a = -1;
-    for (int k = 0; k < 3; ++k)
-      if (std::abs(std::abs(n[k]) - 1.0) <= tol) a = k;
-    if (a < 0 || (axis >= 0 && a != axis)) return -1;
-    axis = a;
-  }
-  return axis;
-}
-
-namespace {
-
-std::vector<std::string> check_case_against_mesh(const CaseConfig& cfg, const PolyMesh& mesh, const MeshGeometry& g,
-                                                 const VelocityGrid& grid) {
+std::vector<std::string> mismatches(const CaseConfig& cfg, const PolyMesh& mesh, const MeshGeometry& g,
+                                    const VelocityGrid& grid) {
   std::vector<std::string> errors;
   for (const Patch& p : mesh.patches)
     if (cfg.find_boundary(p.name) == nullptr) errors.push_back("mesh patch \"" + p.name + "\" has no boundary condition");
@@ -190,6 +231,7 @@ std::vector<std::string> check_case_against_mesh(const CaseConfig& cfg, const Po
     const Patch& patch = mesh.patches[static_cast<std::size_t>(pi)];
     if (patch.count == 0) continue;
     if (b.type == BoundaryType::PeriodicSlab) {
+      // a periodic slab side pairs faces within each cell: their area vectors cancel per cell
       std::map<Index, std::pair<Vec3, double>> per_cell;
       for (Index f = patch.start; f < patch.end(); ++f) {
         auto& e = per_cell[mesh.owner[static_cast<std::size_t>(f)]];
@@ -201,265 +243,312 @@ std::vector<std::string> check_case_against_mesh(const CaseConfig& cfg, const Po
           errors.push_back("periodic patch \"" + b.patch + "\" faces are not paired per cell");
           break;
         }
-      continue;
+    } else if (b.type == BoundaryType::Symmetry) {
+      const int axis = patch_normal_axis(g, patch);
+      if (axis < 0)
+        errors.push_back("symmetry patch \"" + b.patch + "\" normal is not aligned with one velocity axis");
+      else if (!grid.symmetric(axis))
+        errors.push_back("symmetry patch \"" + b.patch + "\": velocity range is not symmetric on its axis");
     }
-    if (b.type != BoundaryType::Symmetry) continue;
-    const int axis = symmetry_patch_axis(g, patch);
-    if (axis < 0)
-      errors.push_back("symmetry patch \"" + b.patch + "\" normal is not aligned with one velocity axis");
-    else if (!grid.symmetric(axis))
-      errors.push_back("symmetry patch \"" + b.patch + "\": velocity range is not symmetric on its axis");
   }
   return errors;
 }
 
-}  // namespace
+class SetupBuilder {
+ public:
+  SetupBuilder(const CaseConfig& cfg, const PolyMesh& mesh, const KineticParams& params)
+      : cfg_(cfg), input_(mesh), grid_(cfg.vn, cfg.vlo, cfg.vhi, cfg.vstretch) {
+    s_.params = params;
+  }
 
-KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh, const KineticParams& params) {
-  KineticSetup s;
-  s.params = params;
-  s.case_name = config.name;
-  s.mesh_file = config.mesh_path();
-  s.gsis_tolerance = config.gsis_tolerance;
-  s.mass_balance_tolerance = config.gsis_mass_balance_tolerance;
-  s.mass_balance_required = config.gsis_mass_balance_required;
-  s.section_positions = config.section_positions;
-  s.section_axis = config.section_axis;
-  s.limiter_mode = params.limiter == "none" ? 0 : (params.limiter_eps == "relative" ? 2 : 1);
-  s.derived = derive_quantities(config);
-  s.length_scale = config.L_ref;
-  const DerivedQuantities& ref = s.derived;
-  const VelocityGrid grid(config.vn, config.vlo, config.vhi, config.vstretch);
-  const MeshGeometry geometry0 = compute_geometry(mesh);
-  {
-    const std::vector<std::string> errors = check_case_against_mesh(config, mesh, geometry0, grid);
+  KineticSetup build() {
+    case_facts();
+    const MeshGeometry input_geometry = compute_geometry(input_);
+    const std::vector<std::string> errors = mismatches(cfg_, input_, input_geometry, grid_);
     if (!errors.empty()) {
       std::string msg = "case and mesh do not match:";
       for (const std::string& e : errors) msg += "\n  " + e;
       throw Error(msg);
     }
+    number_cells(input_geometry);
+    boundary_conditions();
+    maxwell_slots();
+    velocities();
+    cell_scalars();
+    sweep_orders();
+    face_tables();
+    return std::move(s_);
   }
-  long long P_req = params.sweep_blocks;
-  if (P_req == 0) P_req = std::clamp<long long>(mesh.n_cells / 2048, 1, 64);
-  const int P = static_cast<int>(std::min<long long>(P_req, mesh.n_cells));
-  if (params.cell_ordering == "rcm" || params.cell_ordering == "rcb") {
-    if (params.cell_ordering == "rcb")
-      s.cell_new_to_old = rcb_ordering(mesh, geometry0, P, s.block_offsets);
-    else
-      s.cell_new_to_old = rcm_ordering(mesh);
-    s.mesh = renumber_cells(mesh, s.cell_new_to_old);
-    s.cell_old_to_new = invert_permutation(s.cell_new_to_old);
-    s.bface_new_to_old = match_boundary_faces(mesh, s.mesh);
-    s.bface_old_to_new = invert_permutation(s.bface_new_to_old);
-    s.geometry = compute_geometry(s.mesh);
-  } else {
-    s.mesh = mesh;
-    s.geometry = geometry0;
-    s.cell_new_to_old.resize(static_cast<std::size_t>(mesh.n_cells));
-    std::iota(s.cell_new_to_old.begin(), s.cell_new_to_old.end(), 0);
-    s.cell_old_to_new = s.cell_new_to_old;
-    s.bface_new_to_old.resize(static_cast<std::size_t>(mesh.n_boundary_faces()));
-    std::iota(s.bface_new_to_old.begin(), s.bface_new_to_old.end(), 0);
-    s.bface_old_to_new = s.bface_new_to_old;
-  }
-  s.csr = build_cell_faces(s.mesh);
-  s.lsq = build_lsq_coefficients(s.mesh, s.geometry, s.csr, params.lsq_weight_power);
 
-  const Index n_int = s.mesh.n_internal_faces();
-  std::vector<int> mirror_axes;
-  s.closed_domain = true;
-  for (const Patch& patch : s.mesh.patches) {
-    const BoundarySpec* b = config.find_boundary(patch.name);
-    PatchCondition pc;
-    pc.name = patch.name;
-    pc.type = b->type;
-    pc.first_bface = patch.start - n_int;
-    pc.n_faces = patch.count;
-    switch (b->type) {
-      case BoundaryType::MaxwellWall:
-      case BoundaryType::DiffuseWall:
-        pc.accommodation = b->accommodation;
+ private:
+  const CaseConfig& cfg_;
+  const PolyMesh& input_;
+  const VelocityGrid grid_;
+  KineticSetup s_;
+  std::vector<int> mirror_axes_;
+
+  // Case identity, scales and the per-run constants the backend reads.
+  void case_facts() {
+    const KineticParams& p = s_.params;
+    s_.case_name = cfg_.name;
+    s_.mesh_file = cfg_.mesh_path();
+    s_.gsis_tolerance = cfg_.gsis_tolerance;
+    s_.mass_balance_tolerance = cfg_.gsis_mass_balance_tolerance;
+    s_.mass_balance_required = cfg_.gsis_mass_balance_required;
+    s_.section_positions = cfg_.section_positions;
+    s_.section_axis = cfg_.section_axis;
+    s_.derived = derive_quantities(cfg_);
+    s_.length_scale = cfg_.L_ref;
+    s_.limiter_mode = p.limiter == "none" ? 0 : p.limiter_eps == "relative" ? 2 : 1;
+    s_.discrete_eq = p.equilibrium == "discrete";
+    s_.deq_tolerance = p.precision == "fp64" ? 1e-11 : 1e-6;
+    for (int a = 0; a < 3; ++a) {
+      s_.deq_lo[a] = grid_.lo(a);
+      s_.deq_hi[a] = grid_.hi(a);
+      s_.deq_d[a] = grid_.max_spacing(a);
+    }
+    s_.grid_n = cfg_.vn;
+    s_.mu0 = s_.derived.mu_hat0;
+    s_.omega = cfg_.omega;
+    s_.Pr = cfg_.Pr;
+    s_.gamma = cfg_.gamma;
+  }
+
+  // Block count: [kinetic].sweep_blocks, or n_cells / 2048 clamped to 1..64 when it is 0; never more
+  // than the cells. "rcb" numbers cells block by block (bisection + block RCM); "rcm" and "none"
+  // number the whole mesh and cut it into equal contiguous blocks. A renumbered mesh gets its
+  // geometry recomputed (faces may be re-oriented), boundary faces keep their patch order.
+  void number_cells(const MeshGeometry& input_geometry) {
+    const std::string& ordering = s_.params.cell_ordering;
+    const long long wanted = s_.params.sweep_blocks > 0 ? s_.params.sweep_blocks
+                                                        : std::clamp<long long>(input_.n_cells / 2048, 1, 64);
+    const int P = static_cast<int>(std::min<long long>(wanted, input_.n_cells));
+    if (ordering == "none") {
+      s_.mesh = input_;
+      s_.geometry = input_geometry;
+      s_.cell_new_to_old.resize(static_cast<std::size_t>(input_.n_cells));
+      std::iota(s_.cell_new_to_old.begin(), s_.cell_new_to_old.end(), 0);
+      s_.bface_new_to_old.resize(static_cast<std::size_t>(input_.n_boundary_faces()));
+      std::iota(s_.bface_new_to_old.begin(), s_.bface_new_to_old.end(), 0);
+    } else {
+      s_.cell_new_to_old = ordering == "rcb" ? rcb_ordering(input_, input_geometry, P, s_.block_offsets)
+                                             : rcm_ordering(input_);
+      s_.mesh = renumber_cells(input_, s_.cell_new_to_old);
+      s_.bface_new_to_old = match_boundary_faces(input_, s_.mesh);
+      s_.geometry = compute_geometry(s_.mesh);
+    }
+    s_.cell_old_to_new = invert_permutation(s_.cell_new_to_old);
+    s_.bface_old_to_new = invert_permutation(s_.bface_new_to_old);
+    if (ordering != "rcb") s_.block_offsets = partition_contiguous(s_.mesh.n_cells, P);
+    s_.csr = build_cell_faces(s_.mesh);
+    s_.lsq = build_lsq_coefficients(s_.mesh, s_.geometry, s_.csr, s_.params.lsq_weight_power);
+  }
+
+  // One PatchCondition per mesh patch, in mesh order, from its [[boundary]] entry (values made
+  // nondimensional by the reference scales). Inlets and outlets open the domain. A wall with
+  // accommodation < 1 becomes a Maxwell wall; symmetry planes and the axis-aligned faces of
+  // Maxwell walls add mirror axes to the velocity layout.
+  void boundary_conditions() {
+    const DerivedQuantities& ref = s_.derived;
+    const Index first_bface = s_.mesh.n_internal_faces();
+    s_.closed_domain = true;
+    for (const Patch& patch : s_.mesh.patches) {
+      const BoundarySpec& spec = *cfg_.find_boundary(patch.name);
+      PatchCondition pc;
+      pc.name = patch.name;
+      pc.type = spec.type;
+      pc.first_bface = patch.start - first_bface;
+      pc.n_faces = patch.count;
+      const bool wall = spec.type == BoundaryType::DiffuseWall || spec.type == BoundaryType::MaxwellWall;
+      const bool inlet = spec.type == BoundaryType::PressureInlet || spec.type == BoundaryType::MassFlowInlet;
+      if (wall) {
         pc.kind = kBcDiffuseWall;
-        pc.T = *b->temperature / ref.ref_temperature;
-        if (b->velocity) pc.u_wall = *b->velocity / ref.ref_velocity;
-        break;
-      case BoundaryType::MassFlowInlet:
-      case BoundaryType::PressureInlet:
-        pc.target_mass_flow = b->mass_flow.value_or(0.0) / (ref.ref_mass_flux * ref.ref_length * ref.ref_length);
+        pc.accommodation = spec.accommodation;
+        pc.T = *spec.temperature / ref.ref_temperature;
+        if (spec.velocity) pc.u_wall = *spec.velocity / ref.ref_velocity;
+      } else if (inlet) {
         pc.kind = kBcPressure;
-        pc.T = *b->temperature / ref.ref_temperature;
-        pc.p = b->pressure.value_or(ref.p_in) / ref.ref_pressure;
-        pc.zero_tangential = b->inlet_zero_tangential;
-        s.closed_domain = false;
-        break;
-      case BoundaryType::PressureOutlet:
-        pc.p = *b->pressure / ref.ref_pressure;
+        pc.T = *spec.temperature / ref.ref_temperature;
+        pc.p = spec.pressure.value_or(ref.p_in) / ref.ref_pressure;
+        pc.target_mass_flow = spec.mass_flow.value_or(0.0) / (ref.ref_mass_flux * ref.ref_length * ref.ref_length);
+        pc.zero_tangential = spec.inlet_zero_tangential;
+      } else if (spec.type == BoundaryType::PressureOutlet) {
+        pc.p = *spec.pressure / ref.ref_pressure;
         pc.kind = pc.p > 0.0 ? kBcPressure : kBcVacuum;
-        s.closed_domain = false;
-        break;
-      case BoundaryType::Vacuum:
+      } else if (spec.type == BoundaryType::Vacuum) {
         pc.kind = kBcVacuum;
-        s.closed_domain = false;
-        break;
-      case BoundaryType::Symmetry:
+      } else if (spec.type == BoundaryType::Symmetry) {
         pc.kind = kBcSymmetry;
         if (patch.count > 0) {
-          pc.mirror_axis = symmetry_patch_axis(s.geometry, patch);
+          pc.mirror_axis = patch_normal_axis(s_.geometry, patch);
           GSK_CHECK(pc.mirror_axis >= 0, "symmetry patch \"" + patch.name + "\" normal is not axis aligned");
-          mirror_axes.push_back(pc.mirror_axis);
+          mirror_axes_.push_back(pc.mirror_axis);
         }
-        break;
-      case BoundaryType::PeriodicSlab:
+      } else {
         pc.kind = kBcPeriodic;
-        break;
-    }
-    if (pc.accommodation < 1.0) {
-      pc.kind = kBcMaxwellWall;
-      for (int a = 0; a < 3; ++a) GSK_CHECK(grid.n(a) >= 2, "a Maxwell wall needs at least 2 velocity points per axis");
-      for (Index f = patch.start; f < patch.end(); ++f) {
-        const Vec3& n = s.geometry.face_normal[static_cast<std::size_t>(f)];
-        for (int a = 0; a < 3; ++a)
-          if (std::abs(std::abs(n[a]) - 1.0) < 1e-10) {
-            GSK_CHECK(grid.symmetric(a), "a Maxwell wall mirror axis needs a symmetric velocity range");
-            mirror_axes.push_back(a);
-          }
       }
-    }
-    if ((pc.kind == kBcDiffuseWall || pc.kind == kBcMaxwellWall) && norm(pc.u_wall) > 0.0)
-      for (Index f = patch.start; f < patch.end(); ++f) {
-        const double un = dot(pc.u_wall, s.geometry.face_normal[static_cast<std::size_t>(f)]);
-        GSK_CHECK(std::abs(un) <= 1e-9 * norm(pc.u_wall), "wall \"" + patch.name + "\" velocity has a normal component");
+      if (pc.kind == kBcPressure || pc.kind == kBcVacuum) s_.closed_domain = false;
+      if (wall && pc.accommodation < 1.0) {
+        pc.kind = kBcMaxwellWall;
+        for (int a = 0; a < 3; ++a) GSK_CHECK(grid_.n(a) >= 2, "a Maxwell wall needs at least 2 velocity points per axis");
+        for (Index f = patch.start; f < patch.end(); ++f) {
+          const int a = aligned_axis(s_.geometry.face_normal[static_cast<std::size_t>(f)]);
+          if (a < 0) continue;
+          GSK_CHECK(grid_.symmetric(a), "a Maxwell wall mirror axis needs a symmetric velocity range");
+          mirror_axes_.push_back(a);
+        }
       }
-    s.patches.push_back(pc);
-  }
-  s.bface_patch = s.mesh.boundary_face_patches();
-  s.grid_n = config.vn;
-  for (const auto& pc : s.patches)
-    if (pc.kind == kBcMaxwellWall) s.n_maxwell_faces += pc.n_faces;
-  if (s.n_maxwell_faces > 0) {
-    s.maxwell_offset.assign(static_cast<std::size_t>(s.n_bfaces()), -1);
-    s.maxwell_specular.assign(static_cast<std::size_t>(s.n_bfaces()), 0.0);
-    long long slot = 0;
-    for (int b = 0; b < s.n_bfaces(); ++b) {
-      const auto& pc = s.patches[static_cast<std::size_t>(s.bface_patch[static_cast<std::size_t>(b)])];
-      if (pc.kind == kBcMaxwellWall) {
-        s.maxwell_offset[static_cast<std::size_t>(b)] = (slot++) * grid.size();
-        s.maxwell_specular[static_cast<std::size_t>(b)] = 1.0 - pc.accommodation;
-      }
+      if (wall && norm(pc.u_wall) > 0.0)
+        for (Index f = patch.start; f < patch.end(); ++f)
+          GSK_CHECK(std::abs(dot(pc.u_wall, s_.geometry.face_normal[static_cast<std::size_t>(f)])) <= 1e-9 * norm(pc.u_wall),
+                    "wall \"" + patch.name + "\" velocity has a normal component");
+      s_.patches.push_back(pc);
     }
+    s_.bface_patch = s_.mesh.boundary_face_patches();
   }
-  s.discrete_eq = params.equilibrium == "discrete";
-  for (int a = 0; a < 3; ++a) {
-    s.deq_lo[a] = grid.lo(a);
-    s.deq_hi[a] = grid.hi(a);
-    s.deq_d[a] = grid.max_spacing(a);
-  }
-  s.deq_tolerance = params.precision == "fp64" ? 1e-11 : 1e-6;
-  s.sweep_dirs = make_sweep_directions(params.sweep_order);
-  s.velocity = make_velocity_layout(grid, mirror_axes, s.sweep_dirs);
 
-  const double L = s.length_scale;
-  const auto nc = static_cast<std::size_t>(s.mesh.n_cells);
-  s.volume.resize(nc);
-  s.vdt.assign(nc, 0.0);
-  for (std::size_t c = 0; c < nc; ++c) s.volume[c] = s.geometry.cell_volume[c] / (L * L * L);
-  if (params.pseudo_cfl > 0.0 && params.pseudo_time == "velocity") {
-    s.diag_scale = 1.0 + 1.0 / params.pseudo_cfl;
-  } else if (params.pseudo_cfl > 0.0) {
-    for (Index c = 0; c < s.mesh.n_cells; ++c) {
-      double area = 0.0;
-      for (Offset k = s.csr.begin(c); k < s.csr.end(c); ++k)
-        area += s.geometry.face_area[static_cast<std::size_t>(s.csr.faces[static_cast<std::size_t>(k)])];
-      s.vdt[static_cast<std::size_t>(c)] = s.velocity.xi_max * area / (L * L) / params.pseudo_cfl;
+  // Each Maxwell-wall face owns one row of nv reflected values in the backend's face cache; rows
+  // are handed out in boundary-face order. Other faces: offset -1, specular fraction 0.
+  void maxwell_slots() {
+    for (const PatchCondition& pc : s_.patches)
+      if (pc.kind == kBcMaxwellWall) s_.n_maxwell_faces += pc.n_faces;
+    if (s_.n_maxwell_faces == 0) return;
+    const auto nb = static_cast<std::size_t>(s_.n_bfaces());
+    s_.maxwell_offset.assign(nb, -1);
+    s_.maxwell_specular.assign(nb, 0.0);
+    long long rows = 0;
+    for (std::size_t b = 0; b < nb; ++b) {
+      const PatchCondition& pc = s_.patches[static_cast<std::size_t>(s_.bface_patch[b])];
+      if (pc.kind != kBcMaxwellWall) continue;
+      s_.maxwell_offset[b] = rows * grid_.size();
+      s_.maxwell_specular[b] = 1.0 - pc.accommodation;
+      ++rows;
     }
   }
-  if (params.cell_ordering != "rcb") s.block_offsets = partition_contiguous(s.mesh.n_cells, P);
 
-  s.n_sweep_dirs = static_cast<int>(s.sweep_dirs.size());
-  if (s.n_sweep_dirs > 0) {
-    s.sweep_order.resize(static_cast<std::size_t>(s.n_sweep_dirs) * nc);
-    s.sweep_pos.resize(static_cast<std::size_t>(s.n_sweep_dirs) * nc);
-    std::vector<double> key(nc);
-    for (int g = 0; g < s.n_sweep_dirs; ++g) {
-      const Vec3& d = s.sweep_dirs[static_cast<std::size_t>(g)];
-      for (std::size_t c = 0; c < nc; ++c) key[c] = dot(d, s.geometry.cell_center[c]);
-      std::int32_t* order = s.sweep_order.data() + static_cast<std::size_t>(g) * nc;
-      std::int32_t* pos = s.sweep_pos.data() + static_cast<std::size_t>(g) * nc;
-      for (std::size_t b = 0; b + 1 < s.block_offsets.size(); ++b) {
-        const Index c0 = s.block_offsets[b], c1 = s.block_offsets[b + 1];
-        for (Index c = c0; c < c1; ++c) order[c] = c;
-        std::sort(order + c0, order + c1, [&](std::int32_t a, std::int32_t bb) {
-          const double ka = key[static_cast<std::size_t>(a)], kb = key[static_cast<std::size_t>(bb)];
-          return ka != kb ? ka < kb : a < bb;
-        });
+  void velocities() {
+    s_.sweep_dirs = sweep_direction_table(s_.params.sweep_order);
+    s_.n_sweep_dirs = static_cast<int>(s_.sweep_dirs.size());
+    s_.velocity = layout_velocities(grid_, mirror_axes_, s_.sweep_dirs);
+  }
+
+  // Volumes in L_ref^3, the limiter's eps^2 = K^3 V, and the pseudo-time term: with
+  // pseudo_cfl > 0, either V / dt = |xi|_max (sum of the cell's face areas) / cfl per cell
+  // ("global"), or a diagonal scaled by 1 + 1 / cfl ("velocity").
+  void cell_scalars() {
+    const double L = s_.length_scale, K = s_.params.limiter_k;
+    const auto nc = static_cast<std::size_t>(s_.n_cells());
+    s_.volume.resize(nc);
+    s_.eps2.resize(nc);
+    s_.vdt.assign(nc, 0.0);
+    for (std::size_t c = 0; c < nc; ++c) {
+      s_.volume[c] = s_.geometry.cell_volume[c] / (L * L * L);
+      s_.eps2[c] = K * K * K * s_.volume[c];
+    }
+    const double cfl = s_.params.pseudo_cfl;
+    if (!(cfl > 0.0)) return;
+    if (s_.params.pseudo_time == "velocity") {
+      s_.diag_scale = 1.0 + 1.0 / cfl;
+      return;
+    }
+    for (Index c = 0; c < s_.mesh.n_cells; ++c) {
+      double surface = 0.0;
+      for (Offset k = s_.csr.begin(c); k < s_.csr.end(c); ++k)
+        surface += s_.geometry.face_area[static_cast<std::size_t>(s_.csr.faces[static_cast<std::size_t>(k)])];
+      s_.vdt[static_cast<std::size_t>(c)] = s_.velocity.xi_max * surface / (L * L) / cfl;
+    }
+  }
+
+  // Per sweep direction d, every block's cells in ascending d . x_centre (ties: lower internal
+  // index first): sweep_order[d][i] is the i-th cell in sweep order (the cells of block b keep the
+  // index range of block b) and sweep_pos its inverse. sweep_dir_of[p]: the direction velocity p
+  // follows.
+  void sweep_orders() {
+    if (s_.n_sweep_dirs == 0) return;
+    const auto nc = static_cast<std::size_t>(s_.n_cells());
+    s_.sweep_order.resize(static_cast<std::size_t>(s_.n_sweep_dirs) * nc);
+    s_.sweep_pos.resize(s_.sweep_order.size());
+    std::vector<std::pair<double, std::int32_t>> ranked(nc);
+    for (int d = 0; d < s_.n_sweep_dirs; ++d) {
+      const Vec3& dir = s_.sweep_dirs[static_cast<std::size_t>(d)];
+      for (std::size_t c = 0; c < nc; ++c) ranked[c] = {dot(dir, s_.geometry.cell_center[c]), static_cast<std::int32_t>(c)};
+      for (int b = 0; b < s_.n_blocks(); ++b)  // pairs compare by (projection, index)
+        std::sort(ranked.begin() + s_.block_offsets[static_cast<std::size_t>(b)],
+                  ranked.begin() + s_.block_offsets[static_cast<std::size_t>(b) + 1]);
+      std::int32_t* order = &s_.sweep_order[static_cast<std::size_t>(d) * nc];
+      std::int32_t* pos = &s_.sweep_pos[static_cast<std::size_t>(d) * nc];
+      for (std::size_t i = 0; i < nc; ++i) {
+        order[i] = ranked[i].second;
+        pos[static_cast<std::size_t>(ranked[i].second)] = static_cast<std::int32_t>(i);
       }
-      for (std::size_t k = 0; k < nc; ++k) pos[static_cast<std::size_t>(order[k])] = static_cast<std::int32_t>(k);
     }
-    s.sweep_dir_of.resize(static_cast<std::size_t>(s.velocity.nv));
-    for (int p = 0; p < s.velocity.nv; ++p) {
-      const auto pi = static_cast<std::size_t>(p);
-      s.sweep_dir_of[pi] = static_cast<std::uint8_t>(
-          sweep_direction_of(s.sweep_dirs, Vec3{s.velocity.xi_x[pi], s.velocity.xi_y[pi], s.velocity.xi_z[pi]}));
-    }
+    const VelocityLayout& v = s_.velocity;
+    s_.sweep_dir_of.resize(static_cast<std::size_t>(v.nv));
+    for (std::size_t p = 0; p < s_.sweep_dir_of.size(); ++p)
+      s_.sweep_dir_of[p] =
+          static_cast<std::uint8_t>(nearest_direction(s_.sweep_dirs, Vec3{v.xi_x[p], v.xi_y[p], v.xi_z[p]}));
   }
-  s.mu0 = s.derived.mu_hat0;
-  s.omega = config.omega;
-  s.Pr = config.Pr;
-  s.gamma = config.gamma;
 
-  // backend tables (make_csr_entries / make_bface_infos / make_eps2)
-  GSK_CHECK(s.csr.offsets.back() < (Offset{1} << 31), "too many cell->face entries");
-  s.csr_offsets.assign(s.csr.offsets.begin(), s.csr.offsets.end());
-  s.csr_entries.resize(s.csr.faces.size());
-  for (Index c = 0; c < s.mesh.n_cells; ++c) {
-    const Vec3& xc = s.geometry.cell_center[static_cast<std::size_t>(c)];
-    for (Offset k = s.csr.begin(c); k < s.csr.end(c); ++k) {
-      const auto ki = static_cast<std::size_t>(k);
-      const Index f = s.csr.faces[ki];
-      const Index other = s.csr.other[ki];
-      const Vec3& xf = s.geometry.face_center[static_cast<std::size_t>(f)];
-      const Vec3 S = static_cast<double>(s.csr.sign[ki]) * s.geometry.face_area_vector[static_cast<std::size_t>(f)] / (L * L);
-      const Vec3 ds = (xf - xc) / L;
-      Vec3 dother{};
-      if (other != kInvalid) dother = (xf - s.geometry.cell_center[static_cast<std::size_t>(other)]) / L;
-      const Vec3 coef = s.lsq[ki] * L;
-      gsk_csr_entry& e = s.csr_entries[ki];
-      e.other = other;
-      e.bface = other == kInvalid ? f - n_int : -1;
+  // The backend's face tables in L_ref units: per CSR entry (cell i, face f) the outward area
+  // vector, the offsets x_f - x_i and x_f - x_j, and the LSQ coefficient; per boundary face its
+  // owner, kind, storage mirror (the symmetry plane's axis, or the axis an axis-aligned Maxwell
+  // face is normal to), patch, normal, area and offset from the owner's centre.
+  void face_tables() {
+    const double L = s_.length_scale;
+    const Index n_int = s_.mesh.n_internal_faces();
+    const CellFaceCsr& csr = s_.csr;
+    const MeshGeometry& g = s_.geometry;
+    GSK_CHECK(csr.offsets.back() < (Offset{1} << 31), "too many cell->face entries");
+    s_.csr_offsets.assign(csr.offsets.begin(), csr.offsets.end());
+    s_.csr_entries.resize(csr.faces.size());
+    for (Index c = 0; c < s_.mesh.n_cells; ++c)
+      for (Offset k = csr.begin(c); k < csr.end(c); ++k) {
+        const auto e = static_cast<std::size_t>(k);
+        const auto f = static_cast<std::size_t>(csr.faces[e]);
+        const Index nb = csr.other[e];
+        const Vec3 area = static_cast<double>(csr.sign[e]) * g.face_area_vector[f] / (L * L);
+        const Vec3 to_face = (g.face_center[f] - g.cell_center[static_cast<std::size_t>(c)]) / L;
+        const Vec3 from_nb = nb == kInvalid ? Vec3{} : (g.face_center[f] - g.cell_center[static_cast<std::size_t>(nb)]) / L;
+        const Vec3 lsq = s_.lsq[e] * L;
+        gsk_csr_entry& out = s_.csr_entries[e];
+        out.other = nb;
+        out.bface = nb == kInvalid ? static_cast<int32_t>(csr.faces[e] - n_int) : -1;
+        for (int a = 0; a < 3; ++a) {
+          out.s[a] = area[a];
+          out.ds[a] = to_face[a];
+          out.dother[a] = from_nb[a];
+          out.c[a] = lsq[a];
+        }
+      }
+    s_.bfaces.resize(static_cast<std::size_t>(s_.n_bfaces()));
+    for (std::size_t b = 0; b < s_.bfaces.size(); ++b) {
+      const std::size_t f = b + static_cast<std::size_t>(n_int);
+      const int patch = s_.bface_patch[b];
+      const PatchCondition& pc = s_.patches[static_cast<std::size_t>(patch)];
+      const Index owner = s_.mesh.owner[f];
+      const Vec3& n = g.face_normal[f];
+      const Vec3 d = (g.face_center[f] - g.cell_center[static_cast<std::size_t>(owner)]) / L;
+      int mirror_axis = -1;
+      if (pc.kind == kBcSymmetry) mirror_axis = pc.mirror_axis;
+      if (pc.kind == kBcMaxwellWall) mirror_axis = aligned_axis(n);
+      gsk_bface& out = s_.bfaces[b];
+      out.owner = owner;
+      out.kind = pc.kind;
+      out.mirror_xor = mirror_axis >= 0 ? s_.velocity.axis_xor[static_cast<std::size_t>(mirror_axis)] : 0;
+      out.patch = patch;
       for (int a = 0; a < 3; ++a) {
-        e.s[a] = S[a];
-        e.ds[a] = ds[a];
-        e.dother[a] = dother[a];
-        e.c[a] = coef[a];
+        out.n[a] = n[a];
+        out.d[a] = d[a];
       }
+      out.area = g.face_area[f] / (L * L);
     }
   }
-  s.bfaces.resize(static_cast<std::size_t>(s.n_bfaces()));
-  for (Index b = 0; b < s.n_bfaces(); ++b) {
-    const auto f = static_cast<std::size_t>(b + n_int);
-    const int pi = s.bface_patch[static_cast<std::size_t>(b)];
-    const PatchCondition& pc = s.patches[static_cast<std::size_t>(pi)];
-    const Index owner = s.mesh.owner[f];
-    const Vec3 dx = (s.geometry.face_center[f] - s.geometry.cell_center[static_cast<std::size_t>(owner)]) / L;
-    const Vec3& n = s.geometry.face_normal[f];
-    gsk_bface& o = s.bfaces[static_cast<std::size_t>(b)];
-    o.owner = owner;
-    o.kind = pc.kind;
-    o.mirror_xor = pc.kind == kBcSymmetry ? s.velocity.axis_xor[static_cast<std::size_t>(pc.mirror_axis)] : 0;
-    if (pc.kind == kBcMaxwellWall)
-      for (int a = 0; a < 3; ++a)
-        if (std::abs(std::abs(n[a]) - 1.0) < 1e-10) o.mirror_xor = s.velocity.axis_xor[static_cast<std::size_t>(a)];
-    o.patch = pi;
-    for (int a = 0; a < 3; ++a) {
-      o.n[a] = n[a];
-      o.d[a] = dx[a];
-    }
-    o.area = s.geometry.face_area[f] / (L * L);
-  }
-  const double K = params.limiter_k;
-  s.eps2.resize(nc);
-  for (std::size_t c = 0; c < nc; ++c) s.eps2[c] = K * K * K * s.volume[c];
-  return s;
+};
+
+}  // namespace
+
+KineticSetup build_kinetic_setup(const CaseConfig& config, const PolyMesh& mesh, const KineticParams& params) {
+  return SetupBuilder(config, mesh, params).build();
 }
 
 gsk_setup_view KineticSetup::view() const {

@@ -89,11 +89,6 @@ struct VelocityLayout {
   double xi_max = 0.0;
 };
 
-std::vector<Vec3> make_sweep_directions(const std::string& sweep_order);
-int sweep_direction_of(const std::vector<Vec3>& dirs, const Vec3& xi);
-VelocityLayout make_velocity_layout(const VelocityGrid& grid, const std::vector<int>& mirror_axes,
-                                    const std::vector<Vec3>& sweep_dirs);
-int symmetry_patch_axis(const MeshGeometry& g, const Patch& patch, double tolerance = 1e-8);
 
 struct PatchCondition {
   std::string name;
