@@ -1,365 +1,153 @@
-// CIS orchestration (restates src/kinetic/kinetic_solver.cpp:27-722; line references below are
-// to that file). All host arithmetic is FP64.
+// CIS orchestration over a kinetic backend, host FP64 (the reference's KineticSolver,
+// src/kinetic/kinetic_solver.cpp:27-722; the boundary state lives in boundary.cpp).
+//
+// One CIS iteration is prepare() then sweep():
+//   prepare  moments of f -> positivity check -> closed-domain mass rescaling -> eq. (59) epsilon
+//            -> cell parameters (Shakhov target, discrete equilibrium) -> boundary rounds: pressure
+//            face states -> inflow table -> boundary sums -> fluxes, face states, wall densities
+//   sweep    inner_sweeps LU-SGS sweeps (walls refreshed between them), the residual norm of the
+//            first one, and the moments kept for the next epsilon.
 
 #include "solver.hpp"
 
 #include <algorithm>
 #include <chrono>
 
-#include "../kernels/kinetic_math.hpp"
-
 namespace gsk {
 
 namespace {
-
 double now_seconds() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-// Sequential host form of the discrete conservative Maxwellian in FP64 (discrete_maxwell_seq<double>,
-// kinetic_kernels.hpp:499-540), used for boundary inflow states. Returns iterations / 0 / -1.
-int discrete_maxwell_host(const KineticSetup& s, double tol, double rho, double ux, double uy, double uz, double T,
-                          dev::Maxw<double>& out) {
-  dev::DeqGrid g{};
-  for (int a = 0; a < 3; ++a) {
-    g.lo[a] = s.deq_lo[a];
-    g.hi[a] = s.deq_hi[a];
-    g.d[a] = s.deq_d[a];
-  }
-  g.criterion = s.deq_criterion;
-  g.max_iterations = s.deq_max_iterations;
-  g.tolerance = tol;
-  out = dev::maxw_from_state<double>(rho, ux, uy, uz, T);
-  if (dev::deq_resolved(g, ux, uy, uz, T)) return 0;
-  double beta[5];
-  dev::deq_initial_beta(rho, T, beta);
-  double rel_prev = 1e300;
-  const int nv = s.nv();
-  for (int it = 0; it < g.max_iterations; ++it) {
-    const dev::Maxw<double> m = dev::deq_params<double>(beta, ux, uy, uz);
-    double sums[dev::kDeqSums] = {};
-    double t[dev::kDeqSums];
-    for (int p = 0; p < nv; ++p) {
-      const double x = s.velocity.xi_x[static_cast<std::size_t>(p)], y = s.velocity.xi_y[static_cast<std::size_t>(p)],
-                   z = s.velocity.xi_z[static_cast<std::size_t>(p)];
-      double e = dev::maxw_eval(m, x, y, z);
-      if (!s.velocity.weight_rel.empty()) e *= s.velocity.weight_rel[static_cast<std::size_t>(p)];  // stretched grid
-      dev::deq_terms<double>(e, 0.0, x - ux, y - uy, z - uz, t);
-      for (int k = 0; k < dev::kDeqSums; ++k) sums[k] += t[k];
-    }
-    double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
-    double rel = 0.0;
-    const int r = dev::deq_newton_step(sums, s.velocity.weight, rho, T, g.tolerance, b, rel);
-    if (dev::deq_accept(r, rel, rel_prev, it + 1 == g.max_iterations)) {
-      out = m;
-      return it + 1;
-    }
-    if (r < 0) break;
-    for (int k = 0; k < 5; ++k) beta[k] = b[k];
-    rel_prev = rel;
-  }
-  out = dev::maxw_from_state<double>(rho, ux, uy, uz, T);
-  return -1;
-}
-
 }  // namespace
 
 KineticSolver::KineticSolver(const KineticSetup& setup, std::unique_ptr<Backend> backend)
-    : s_(setup), patches_(setup.patches), backend_(std::move(backend)) {
+    : s_(setup), backend_(std::move(backend)), bnd_(setup) {
   mu0_ = setup.mu0;
   boundary_iterations_ = setup.params.boundary_iterations;
   const auto nc = static_cast<std::size_t>(s_.n_cells());
-  const auto nb = static_cast<std::size_t>(s_.n_bfaces());
   moments_.assign(nc * 15, 0.0);
   prev_moments_.assign(nc * 15, 0.0);
-  bsums_.assign(nb * 20, 0.0);
-  inflow_.assign(nb, gsk_inflow_state{0, 0, 0, 0, 0, 1.0});
-  F_plus_.assign(nb, {});
-  F_minus_.assign(nb, {});
-  face_state_.assign(nb, {});
+  bsums_.assign(static_cast<std::size_t>(s_.n_bfaces()) * 20, 0.0);
   residual_cell_.assign(nc, 0.0);
-  init_static_inflow();
-}
-
-// discrete-equilibrium inflow: replace (rho, u, T) by the Maxwellian whose discrete moments match (:103-114)
-void KineticSolver::correct_inflow(gsk_inflow_state& in) const {
-  if (!s_.discrete_eq || !(in.rho > 0.0) || !(in.T > 0.0)) return;
-  dev::DeqGrid g{};
-  for (int a = 0; a < 3; ++a) {
-    g.lo[a] = s_.deq_lo[a];
-    g.hi[a] = s_.deq_hi[a];
-    g.d[a] = s_.deq_d[a];
-  }
-  g.criterion = s_.deq_criterion;
-  if (dev::deq_resolved(g, in.ux, in.uy, in.uz, in.T)) return;
-  dev::Maxw<double> m;
-  const int r = discrete_maxwell_host(s_, inflow_tol_, in.rho, in.ux, in.uy, in.uz, in.T, m);
-  if (r > 0) {
-    // maxwell_params_to_state (kinetic_kernels.hpp:543-552)
-    in.T = 1.0 / (2.0 * m.B);
-    const double tpt = 2.0 * kPi * in.T;
-    in.rho = m.A * tpt * std::sqrt(tpt);
-    in.ux = m.ux;
-    in.uy = m.uy;
-    in.uz = m.uz;
-  }
-}
-
-void KineticSolver::init_static_inflow() {  // (:117-142)
-  wall_cache_patch_ = -1;
-  for (Index b = 0; b < s_.n_bfaces(); ++b) {
-    const PatchCondition& pc = patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])];
-    gsk_inflow_state& in = inflow_[static_cast<std::size_t>(b)];
-    if (pc.kind == kBcDiffuseWall || pc.kind == kBcMaxwellWall) {
-      in.rho = 1.0;
-      in.ux = pc.u_wall.x;
-      in.uy = pc.u_wall.y;
-      in.uz = pc.u_wall.z;
-      in.T = pc.T;
-      in.scale = 1.0;
-      // every face of a wall patch has the same unit-density state: correct it once per patch
-      const int pi = s_.bface_patch[static_cast<std::size_t>(b)];
-      if (wall_cache_patch_ != pi) {
-        wall_cache_ = in;
-        correct_inflow(wall_cache_);
-        wall_cache_patch_ = pi;
-      }
-      in = wall_cache_;
-    } else if (pc.kind == kBcPressure) {
-      in.scale = 1.0;
-      in.rho = pc.p / pc.T;
-      in.T = pc.T;
-    } else {
-      in = gsk_inflow_state{0, 0, 0, 0, 0, 1.0};
-    }
-  }
-  if (s_.n_maxwell_faces > 0) {
-    wall_unit_moments_.assign(static_cast<std::size_t>(s_.n_bfaces()), {});
-    for (Index b = 0; b < s_.n_bfaces(); ++b)
-      if (patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])].kind == kBcMaxwellWall)
-        wall_unit_moments_[static_cast<std::size_t>(b)] = compute_wall_unit_sums(b);
-  }
+  bnd_.reset(inflow_tol_);
 }
 
 void KineticSolver::reset_after_initialization() {
   ++version_;
   iteration_ = 0;
   has_prev_ = false;
-  has_face_state_ = false;
   has_mass_target_ = false;
-  init_static_inflow();
+  bnd_.reset(inflow_tol_);
 }
 
-void KineticSolver::init_wall_scale_from_cells() {  // (:156-164)
-  const Index n_int = s_.mesh.n_internal_faces();
-  for (Index b = 0; b < s_.n_bfaces(); ++b) {
-    const PatchCondition& pc = patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])];
-    if (pc.kind != kBcDiffuseWall && pc.kind != kBcMaxwellWall) continue;
-    const Index c = s_.mesh.owner[static_cast<std::size_t>(b + n_int)];
-    inflow_[static_cast<std::size_t>(b)].scale =
-        pc.accommodation * cell_value(moments_, c, 0) * std::sqrt(cell_value(moments_, c, 4) / pc.T);
+// Moments of the current f; every cell must keep a positive, finite density and temperature.
+void KineticSolver::read_moments() {
+  backend_->compute_moments(moments_.data(), moments_.size());
+  Index bad = 0, first = -1;
+  for (Index c = 0; c < s_.n_cells(); ++c) {
+    const double* m = &moments_[static_cast<std::size_t>(c) * 15];
+    const bool ok = m[0] > 0.0 && m[4] > 0.0 && std::isfinite(m[0]) && std::isfinite(m[4]);
+    if (ok) continue;
+    if (bad++ == 0) first = c;
+  }
+  if (bad == 0) return;
+  const double* m = &moments_[static_cast<std::size_t>(first) * 15];
+  throw Error(strf("iteration %lld: %d cells have non-positive density or temperature (e.g. original cell %d, rho = "
+                   "%.4g, T = %.4g)",
+                   iteration_, bad, s_.cell_new_to_old[static_cast<std::size_t>(first)], m[0], m[4]));
+}
+
+// A closed domain keeps the mass of its first prepare: f and the density-proportional moments
+// (rho and p, stress, heat flux) are scaled by target / current mass.
+void KineticSolver::rescale_closed_domain() {
+  double mass = 0.0;
+  for (Index c = 0; c < s_.n_cells(); ++c)
+    mass += moments_[static_cast<std::size_t>(c) * 15] * s_.volume[static_cast<std::size_t>(c)];
+  info_.total_mass = mass;
+  if (!s_.closed_domain) return;
+  if (!has_mass_target_) {
+    mass_target_ = mass;
+    has_mass_target_ = true;
+  }
+  const double scale = mass_target_ / mass;
+  info_.mass_scale = scale;
+  if (scale == 1.0) return;
+  backend_->scale_distribution(scale);
+  for (std::size_t c = 0; c < static_cast<std::size_t>(s_.n_cells()); ++c) {
+    double* m = &moments_[c * 15];
+    m[0] *= scale;
+    for (int k = 5; k < 15; ++k) m[k] *= scale;
   }
 }
 
-void KineticSolver::update_pressure_inflow() {  // (:172-260)
-  const Index n_int = s_.mesh.n_internal_faces();
-  const double K = s_.params.limiter_k;
-  const bool limit = s_.limiter_mode != 0;
-  std::vector<Index> pending;
-  for (const PatchCondition& pc : patches_) {
-    if (pc.kind != kBcPressure) continue;
-    for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) {
-      const auto f = static_cast<std::size_t>(b + n_int);
-      const Index ci = s_.mesh.owner[f];
-      auto state_of = [&](Index c) {
-        return std::array<double, 5>{cell_value(moments_, c, 5), cell_value(moments_, c, 1), cell_value(moments_, c, 2),
-                                     cell_value(moments_, c, 3), cell_value(moments_, c, 4)};
-      };
-      const std::array<double, 5> wi = state_of(ci);
-      std::array<double, 5> wmax = wi, wmin = wi;
-      std::array<Vec3, 5> grad{};
-      for (Offset k = s_.csr.begin(ci); k < s_.csr.end(ci); ++k) {
-        const auto ki = static_cast<std::size_t>(k);
-        std::array<double, 5> wk;
-        if (s_.csr.other[ki] != kInvalid) {
-          wk = state_of(s_.csr.other[ki]);
-        } else {
-          const auto bk = static_cast<std::size_t>(s_.csr.faces[ki] - n_int);
-          const int kind = patches_[static_cast<std::size_t>(s_.bface_patch[bk])].kind;
-          const auto& fs = face_state_[bk];
-          if (has_face_state_ && kind != kBcPeriodic && fs[0] > 0.0 && fs[4] > 0.0)
-            wk = {fs[0] * fs[4], fs[1], fs[2], fs[3], fs[4]};
-          else
-            wk = wi;
-        }
-        for (int q = 0; q < 5; ++q) {
-          const auto qi = static_cast<std::size_t>(q);
-          const double d = wk[qi] - wi[qi];
-          grad[qi] += d * s_.lsq[ki];
-          wmax[qi] = std::max(wmax[qi], wk[qi]);
-          wmin[qi] = std::min(wmin[qi], wk[qi]);
-        }
-      }
-      const Vec3 dxb = s_.geometry.face_center[f] - s_.geometry.cell_center[static_cast<std::size_t>(ci)];
-      const double eps2 = K * K * K * s_.volume[static_cast<std::size_t>(ci)];
-      std::array<double, 5> wib{};
-      for (int q = 0; q < 5; ++q) {
-        const auto qi = static_cast<std::size_t>(q);
-        double phi = 1.0;
-        if (limit) {
-          const double dmx = wmax[qi] - wi[qi], dmn = wmin[qi] - wi[qi];
-          for (Offset k = s_.csr.begin(ci); k < s_.csr.end(ci); ++k) {
-            const auto ki = static_cast<std::size_t>(k);
-            const Index fk = s_.csr.faces[ki];
-            if (s_.csr.other[ki] == kInvalid &&
-                patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(fk - n_int)])].kind == kBcPeriodic)
-              continue;
-            const double d2 = dot(grad[qi], s_.geometry.face_center[static_cast<std::size_t>(fk)] -
-                                                s_.geometry.cell_center[static_cast<std::size_t>(ci)]);
-            phi = std::min(phi, dev::venkat_face<double>(d2, dmx, dmn, eps2));
-          }
-        }
-        wib[qi] = wi[qi] + phi * dot(grad[qi], dxb);
-      }
-      if (!(wib[0] > 0.0 && wib[4] > 0.0)) wib = wi;
-      if (s_.params.pressure_boundary_reconstruction == "cell") wib = wi;
-      const double p_ib = wib[0];
-      const Vec3 u_ib{wib[1], wib[2], wib[3]};
-      const double T_ib = wib[4];
-      const double rho_ib = p_ib / T_ib;
-      const double cs = std::sqrt(s_.gamma * T_ib);
-      const Vec3& n = s_.geometry.face_normal[f];
-      double p_b, T_b;
-      if (pc.type == BoundaryType::PressureInlet || pc.type == BoundaryType::MassFlowInlet) {
-        p_b = pc.p;
-        T_b = pc.T;
-      } else {
-        p_b = pc.p;
-        T_b = T_ib;
-      }
-      const double du = (p_ib - p_b) / (rho_ib * cs);
-      gsk_inflow_state& in = inflow_[static_cast<std::size_t>(b)];
-      in.scale = 1.0;
-      in.rho = p_b / T_b;
-      const Vec3 u = pc.zero_tangential ? (dot(u_ib, n) + du) * n : u_ib + du * n;
-      in.ux = u.x;
-      in.uy = u.y;
-      in.uz = u.z;
-      in.T = T_b;
-      pending.push_back(b);
-    }
-  }
-  // discrete-equilibrium correction of the new inflow states: independent per face, on the
-  // backend's device when it provides it, else on the host cores
-  if (!s_.discrete_eq || pending.empty()) return;
-  std::vector<gsk_inflow_state> batch(pending.size());
-  for (std::size_t i = 0; i < pending.size(); ++i) batch[i] = inflow_[static_cast<std::size_t>(pending[i])];
-  if (!backend_->correct_inflow(batch.data(), batch.size())) {
-#pragma omp parallel for schedule(dynamic, 4)
-    for (std::size_t i = 0; i < batch.size(); ++i) correct_inflow(batch[i]);
-  }
-  for (std::size_t i = 0; i < pending.size(); ++i) inflow_[static_cast<std::size_t>(pending[i])] = batch[i];
-}
-
-std::array<double, 10> KineticSolver::compute_wall_unit_sums(Index b) const {  // (:263-277)
-  std::array<double, 10> out{};
-  const Vec3& n = s_.geometry.face_normal[static_cast<std::size_t>(b + s_.mesh.n_internal_faces())];
-  const gsk_inflow_state& in = inflow_[static_cast<std::size_t>(b)];
-  const dev::Maxw<double> m = dev::maxw_from_state<double>(in.rho, in.ux, in.uy, in.uz, in.T);
-  for (int p = 0; p < s_.nv(); ++p) {
-    const Vec3 xi{s_.velocity.xi_x[static_cast<std::size_t>(p)], s_.velocity.xi_y[static_cast<std::size_t>(p)],
-                  s_.velocity.xi_z[static_cast<std::size_t>(p)]};
-    const double un = dot(xi, n);
-    if (un >= 0.0) continue;
-    double wf = s_.velocity.weight * dev::maxw_eval(m, xi.x, xi.y, xi.z);
-    if (!s_.velocity.weight_rel.empty()) wf *= s_.velocity.weight_rel[static_cast<std::size_t>(p)];  // stretched grid
-    const double psi[5] = {1.0, xi.x, xi.y, xi.z, 0.5 * norm2(xi)};
-    for (int k = 0; k < 5; ++k) {
-      out[static_cast<std::size_t>(k)] += wf * un * psi[k];
-      out[static_cast<std::size_t>(5 + k)] += wf * psi[k];
-    }
-  }
-  return out;
-}
-
-void KineticSolver::process_boundary_sums() {  // (:280-335)
-  const Index nb = s_.n_bfaces();
-  const double L = s_.length_scale;
-  const Index n_int = s_.mesh.n_internal_faces();
-  patch_mass_flow_.assign(patches_.size(), 0.0);
-  for (Index b = 0; b < nb; ++b) {
-    const auto bi = static_cast<std::size_t>(b);
-    const double* sm = &bsums_[bi * 20];
-    const int pi = s_.bface_patch[bi];
-    const PatchCondition& pc = patches_[static_cast<std::size_t>(pi)];
-    const double area = s_.geometry.face_area[static_cast<std::size_t>(b + n_int)] / (L * L);
-    double in_scale = 1.0;
-    if (pc.kind == kBcDiffuseWall) {
-      const double jp = sm[0], jm = sm[5];
-      const double rho_g = jm < 0.0 ? std::max(0.0, -jp / jm) : 0.0;
-      const double w = s_.params.wall_relaxation;
-      inflow_[bi].scale = w == 1.0 ? rho_g : w * rho_g + (1.0 - w) * inflow_[bi].scale;
-      in_scale = rho_g;
-    } else if (pc.kind == kBcVacuum) {
-      in_scale = 0.0;
-    }
-    std::array<double, 10> unit{};
-    double delta = 0.0;
-    if (pc.kind == kBcMaxwellWall) {
-      unit = wall_unit_moments_[bi];
-      GSK_CHECK(unit[0] < 0.0, "Maxwell wall has no incoming velocity points");
-      delta = -(sm[0] + sm[5]) / unit[0];
-      const double next = inflow_[bi].scale + delta;
-      GSK_CHECK(next >= -1e-8, "Maxwell wall diffuse density became negative; refine or widen the velocity grid");
-      inflow_[bi].scale = std::max(0.0, next);
-    }
-    std::array<double, 5> mom{};
-    for (int k = 0; k < 5; ++k) {
-      const auto ks = static_cast<std::size_t>(k);
-      F_plus_[bi][ks] = area * sm[k];
-      if (pc.kind == kBcMaxwellWall) {
-        F_minus_[bi][ks] = area * (sm[5 + k] + delta * unit[ks]);
-        mom[ks] = sm[10 + k] + sm[15 + k] + delta * unit[5 + ks];
-      } else {
-        F_minus_[bi][ks] = area * in_scale * sm[5 + k];
-        mom[ks] = sm[10 + k] + in_scale * sm[15 + k];
-      }
-    }
-    auto& fs = face_state_[bi];
-    if (mom[0] > 0.0) {
-      const Vec3 u{mom[1] / mom[0], mom[2] / mom[0], mom[3] / mom[0]};
-      fs = {mom[0], u.x, u.y, u.z, (2.0 * mom[4] / mom[0] - norm2(u)) / 3.0};
-    } else {
-      fs = {0.0, 0.0, 0.0, 0.0, 0.0};
-    }
-    patch_mass_flow_[static_cast<std::size_t>(pi)] += F_plus_[bi][0] + F_minus_[bi][0];
-  }
-  has_face_state_ = true;
-}
-
-void KineticSolver::compute_epsilon() {  // eq. (59) (:338-363)
-  double num[3] = {0, 0, 0}, den[3] = {0, 0, 0}, vol = 0.0;
+// eq. (59): epsilon = max over (rho, u, T) of the volume-weighted relative L2 change since the last
+// iteration; the velocity change is measured against max(|u|, convergence_u_floor).
+void KineticSolver::measure_convergence() {
+  double change[3] = {0, 0, 0}, size[3] = {0, 0, 0}, vol = 0.0;
   for (Index c = 0; c < s_.n_cells(); ++c) {
     const double V = s_.volume[static_cast<std::size_t>(c)];
-    const double* a = &moments_[static_cast<std::size_t>(c) * 15];
-    const double* b = &prev_moments_[static_cast<std::size_t>(c) * 15];
-    num[0] += (a[0] - b[0]) * (a[0] - b[0]) * V;
-    den[0] += b[0] * b[0] * V;
-    const double du2 = (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]) + (a[3] - b[3]) * (a[3] - b[3]);
-    num[1] += du2 * V;
-    den[1] += (b[1] * b[1] + b[2] * b[2] + b[3] * b[3]) * V;
-    num[2] += (a[4] - b[4]) * (a[4] - b[4]) * V;
-    den[2] += b[4] * b[4] * V;
+    const double* now = &moments_[static_cast<std::size_t>(c) * 15];
+    const double* was = &prev_moments_[static_cast<std::size_t>(c) * 15];
+    const double dr = now[0] - was[0], dT = now[4] - was[4];
+    const double du[3] = {now[1] - was[1], now[2] - was[2], now[3] - was[3]};
+    change[0] += dr * dr * V;
+    size[0] += was[0] * was[0] * V;
+    change[1] += (du[0] * du[0] + du[1] * du[1] + du[2] * du[2]) * V;
+    size[1] += (was[1] * was[1] + was[2] * was[2] + was[3] * was[3]) * V;
+    change[2] += dT * dT * V;
+    size[2] += was[4] * was[4] * V;
     vol += V;
   }
   const double floor = s_.params.convergence_u_floor;
-  den[1] = std::max(den[1], floor * floor * vol);
-  auto ratio = [](double n, double d) { return d > 0.0 ? std::sqrt(n / d) : (n > 0.0 ? 1.0 : 0.0); };
-  info_.eps_rho = ratio(num[0], den[0]);
-  info_.eps_u = ratio(num[1], den[1]);
-  info_.eps_T = ratio(num[2], den[2]);
+  size[1] = std::max(size[1], floor * floor * vol);
+  auto rel = [](double d, double n) { return n > 0.0 ? std::sqrt(d / n) : (d > 0.0 ? 1.0 : 0.0); };
+  info_.eps_rho = rel(change[0], size[0]);
+  info_.eps_u = rel(change[1], size[1]);
+  info_.eps_T = rel(change[2], size[2]);
   info_.epsilon = std::max({info_.eps_rho, info_.eps_u, info_.eps_T});
   info_.has_epsilon = 1;
+}
+
+// Volume means of the heat flux and the stress (reported per iteration).
+void KineticSolver::iteration_statistics() {
+  double vol = 0.0;
+  for (Index c = 0; c < s_.n_cells(); ++c) {
+    const double V = s_.volume[static_cast<std::size_t>(c)];
+    const double* m = &moments_[static_cast<std::size_t>(c) * 15];
+    for (int k = 0; k < 3; ++k) info_.q_mean[k] += V * m[12 + k];
+    for (int k = 0; k < 6; ++k) info_.sigma_mean[k] += V * m[6 + k];
+    vol += V;
+  }
+  for (double& x : info_.q_mean) x /= vol;
+  for (double& x : info_.sigma_mean) x /= vol;
+}
+
+// Discrete-equilibrium correction of new pressure-face states: on the backend's device when it
+// offers it, else per face on the host cores.
+void KineticSolver::correct_pressure_states(const std::vector<Index>& faces) {
+  if (!s_.discrete_eq || faces.empty()) return;
+  std::vector<gsk_inflow_state> batch(faces.size());
+  for (std::size_t i = 0; i < faces.size(); ++i) batch[i] = bnd_.inflow[static_cast<std::size_t>(faces[i])];
+  if (!backend_->correct_inflow(batch.data(), batch.size())) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (std::size_t i = 0; i < batch.size(); ++i) discrete_inflow_state(s_, inflow_tol_, batch[i]);
+  }
+  for (std::size_t i = 0; i < faces.size(); ++i) bnd_.inflow[static_cast<std::size_t>(faces[i])] = batch[i];
+}
+
+// boundary_iterations rounds: pressure-face states from the moments (and the previous round's face
+// states), then the boundary sums with those states, then what the sums imply (wall densities,
+// fluxes, face states); the backend always holds the latest inflow table.
+void KineticSolver::boundary_rounds() {
+  if (!bnd_.has_face_states()) bnd_.seed_wall_densities(moments_);
+  for (long long round = 0; round < boundary_iterations_; ++round) {
+    correct_pressure_states(bnd_.update_pressure_faces(moments_));
+    backend_->set_inflow(bnd_.inflow.data(), bnd_.inflow.size());
+    backend_->boundary_sums(bsums_.data(), bsums_.size());
+    bnd_.absorb(bsums_);
+    backend_->set_inflow(bnd_.inflow.data(), bnd_.inflow.size());
+  }
 }
 
 const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
@@ -368,51 +156,10 @@ const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
   info_ = gsk_iteration_info{};
   info_.iteration = iteration_;
   info_.mass_scale = 1.0;
-  backend_->compute_moments(moments_.data(), moments_.size());
-  Index bad = 0, first_bad = -1;
-  for (Index c = 0; c < s_.n_cells(); ++c) {
-    const double rho = cell_value(moments_, c, 0), T = cell_value(moments_, c, 4);
-    if (!(rho > 0.0) || !(T > 0.0) || !std::isfinite(rho) || !std::isfinite(T)) {
-      if (bad == 0) first_bad = c;
-      ++bad;
-    }
-  }
-  GSK_CHECK(bad == 0, strf("iteration %lld: %d cells have non-positive density or temperature (e.g. original cell %d, "
-                           "rho = %.4g, T = %.4g)",
-                           iteration_, bad, bad ? s_.cell_new_to_old[static_cast<std::size_t>(first_bad)] : -1,
-                           bad ? cell_value(moments_, first_bad, 0) : 0.0, bad ? cell_value(moments_, first_bad, 4) : 0.0));
-  double mass = 0.0;
-  for (Index c = 0; c < s_.n_cells(); ++c) mass += cell_value(moments_, c, 0) * s_.volume[static_cast<std::size_t>(c)];
-  info_.total_mass = mass;
-  if (s_.closed_domain) {
-    if (!has_mass_target_) {
-      mass_target_ = mass;
-      has_mass_target_ = true;
-    }
-    const double scale = mass_target_ / mass;
-    if (scale != 1.0) {
-      backend_->scale_distribution(scale);
-      for (Index c = 0; c < s_.n_cells(); ++c) {
-        double* m = &moments_[static_cast<std::size_t>(c) * 15];
-        m[0] *= scale;
-        for (int k = 5; k < 15; ++k) m[k] *= scale;
-      }
-    }
-    info_.mass_scale = scale;
-  }
-  if (has_prev_) compute_epsilon();
-  {
-    double vol = 0.0;
-    for (Index c = 0; c < s_.n_cells(); ++c) {
-      const double V = s_.volume[static_cast<std::size_t>(c)];
-      const double* m = &moments_[static_cast<std::size_t>(c) * 15];
-      for (int k = 0; k < 3; ++k) info_.q_mean[k] += V * m[12 + k];
-      for (int k = 0; k < 6; ++k) info_.sigma_mean[k] += V * m[6 + k];
-      vol += V;
-    }
-    for (double& x : info_.q_mean) x /= vol;
-    for (double& x : info_.sigma_mean) x /= vol;
-  }
+  read_moments();
+  rescale_closed_domain();
+  if (has_prev_) measure_convergence();
+  iteration_statistics();
   backend_->update_cell_params();
   if (s_.discrete_eq) {
     const gsk_deq_counts dc = backend_->deq_counts();
@@ -420,87 +167,56 @@ const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
     info_.deq_failed = dc.failed;
     info_.deq_max_iterations = dc.max_iterations;
   }
-  if (!has_face_state_) init_wall_scale_from_cells();
-  update_pressure_inflow();
-  backend_->set_inflow(inflow_.data(), inflow_.size());
-  backend_->boundary_sums(bsums_.data(), bsums_.size());
-  process_boundary_sums();
-  backend_->set_inflow(inflow_.data(), inflow_.size());
-  for (long long b = 1; b < boundary_iterations_; ++b) {
-    update_pressure_inflow();
-    backend_->set_inflow(inflow_.data(), inflow_.size());
-    backend_->boundary_sums(bsums_.data(), bsums_.size());
-    process_boundary_sums();
-    backend_->set_inflow(inflow_.data(), inflow_.size());
-  }
+  boundary_rounds();
   prepared_version_ = version_;
   info_.seconds = now_seconds() - t0;
   return info_;
 }
 
-void KineticSolver::refresh_wall_inflow() {  // (:456-480)
-  // only the wall faces' sums are read here
+// Between inner sweeps only the wall densities follow f (kinetic_solver.cpp:456-480): on the device
+// when the backend keeps them there, else from the wall faces' boundary sums on the host.
+void KineticSolver::refresh_wall_inflow() {
   if (wall_faces_.empty()) {
     for (Index b = 0; b < s_.n_bfaces(); ++b) {
-      const int k = patches_[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(b)])].kind;
-      if (k == kBcDiffuseWall || k == kBcMaxwellWall) wall_faces_.push_back(b);
+      const BoundaryKind k = bnd_.patch_of(b).kind;
+      if (k != kBcDiffuseWall && k != kBcMaxwellWall) continue;
+      const bool maxwell = k == kBcMaxwellWall;
+      wall_faces_.push_back(b);
+      wall_maxwell_.push_back(maxwell ? 1 : 0);
+      wall_unit0_.push_back(maxwell ? bnd_.wall_unit[static_cast<std::size_t>(b)][0] : 0.0);
     }
   }
   if (wall_faces_.empty()) return;
-  // on the device when the backend keeps the scales there (no host round trip per inner sweep)
-  wall_maxwell_.resize(wall_faces_.size());
-  wall_unit0_.resize(wall_faces_.size());
-  for (std::size_t i = 0; i < wall_faces_.size(); ++i) {
-    const auto bi = static_cast<std::size_t>(wall_faces_[i]);
-    const bool mw = patches_[static_cast<std::size_t>(s_.bface_patch[bi])].kind == kBcMaxwellWall;
-    wall_maxwell_[i] = mw ? 1 : 0;
-    wall_unit0_[i] = mw ? wall_unit_moments_[bi][0] : 0.0;
-  }
   if (backend_->refresh_wall_inflow(wall_faces_.data(), wall_maxwell_.data(), wall_unit0_.data(), wall_faces_.size(),
                                     s_.params.wall_relaxation)) {
     wall_scales_on_device_ = true;
     return;
   }
   backend_->boundary_sums_subset(wall_faces_.data(), wall_faces_.size(), bsums_.data(), bsums_.size());
-  bool any = false;
-  for (Index b = 0; b < s_.n_bfaces(); ++b) {
-    const auto bi = static_cast<std::size_t>(b);
-    const PatchCondition& pc = patches_[static_cast<std::size_t>(s_.bface_patch[bi])];
-    if (pc.kind != kBcDiffuseWall && pc.kind != kBcMaxwellWall) continue;
-    const double* sm = &bsums_[bi * 20];
-    if (pc.kind == kBcMaxwellWall) {
-      const auto& unit = wall_unit_moments_[bi];
-      const double next = inflow_[bi].scale - (sm[0] + sm[5]) / unit[0];
-      GSK_CHECK(next >= -1e-8, "Maxwell wall diffuse density became negative; refine the velocity grid");
-      inflow_[bi].scale = std::max(0.0, next);
-      any = true;
-      continue;
-    }
-    const double jp = sm[0], jm = sm[5];
-    const double rho_g = jm < 0.0 ? std::max(0.0, -jp / jm) : 0.0;
-    const double w = s_.params.wall_relaxation;
-    inflow_[bi].scale = w == 1.0 ? rho_g : w * rho_g + (1.0 - w) * inflow_[bi].scale;
-    any = true;
-  }
-  if (any) backend_->set_inflow(inflow_.data(), inflow_.size());
+  bnd_.refresh_walls(bsums_, wall_faces_);
+  backend_->set_inflow(bnd_.inflow.data(), bnd_.inflow.size());
 }
 
-void KineticSolver::update_mass_flow_pressure() {  // (:482-501)
+// Mass-flow inlets: the patch pressure moves towards the target flow by a relaxed Newton step on
+// the free-molecular flow sensitivity D = sum_faces A (1 / sqrt(2 pi T_in) + 1 / (2 sqrt(gamma T_c))),
+// limited to 10% of p per iteration (kinetic_solver.cpp:482-501).
+void KineticSolver::update_mass_flow_pressure() {
   prepare();
   bool changed = false;
-  for (std::size_t pi = 0; pi < patches_.size(); ++pi) {
-    PatchCondition& pc = patches_[pi];
+  const double L2 = s_.length_scale * s_.length_scale;
+  std::vector<PatchCondition>& patches = bnd_.patches();
+  for (std::size_t pi = 0; pi < patches.size(); ++pi) {
+    PatchCondition& pc = patches[pi];
     if (pc.type != BoundaryType::MassFlowInlet) continue;
     double D = 0.0;
     for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) {
       const auto f = static_cast<std::size_t>(b + s_.mesh.n_internal_faces());
-      const double T = cell_value(moments_, s_.mesh.owner[f], 4);
-      D += s_.geometry.face_area[f] / (s_.length_scale * s_.length_scale) *
-           (1.0 / std::sqrt(2.0 * kPi * pc.T) + 0.5 / std::sqrt(s_.gamma * T));
+      const double T_cell = moments_[static_cast<std::size_t>(s_.mesh.owner[f]) * 15 + 4];
+      D += s_.geometry.face_area[f] / L2 * (1.0 / std::sqrt(2.0 * kPi * pc.T) + 0.5 / std::sqrt(s_.gamma * T_cell));
     }
     GSK_CHECK(D > 0.0, "mass-flow inlet has zero area");
-    const double dp = 0.2 * (pc.target_mass_flow + patch_mass_flow_[pi]) / D;
-    pc.p = std::max(1e-12, pc.p + std::clamp(dp, -0.1 * pc.p, 0.1 * pc.p));
+    const double step = 0.2 * (pc.target_mass_flow + bnd_.patch_flow[pi]) / D;
+    pc.p = std::max(1e-12, pc.p + std::clamp(step, -0.1 * pc.p, 0.1 * pc.p));
     changed = true;
   }
   if (changed) {
@@ -514,17 +230,19 @@ const gsk_iteration_info& KineticSolver::sweep(bool update_pressure) {  // (:503
   if (update_pressure) update_mass_flow_pressure();
   const double t0 = now_seconds();
   backend_->sweep(residual_cell_.data(), residual_cell_.size());
-  for (long long m = 1; m < s_.params.inner_sweeps; ++m) {
+  for (long long inner = 1; inner < s_.params.inner_sweeps; ++inner) {
     if (s_.params.inner_wall_update) refresh_wall_inflow();
     inner_residual_.resize(residual_cell_.size());
     backend_->sweep_unreported(inner_residual_.data(), inner_residual_.size());
   }
-  if (wall_scales_on_device_) {  // the host copy of the wall scales catches up once per iteration
+  if (wall_scales_on_device_) {  // the host copy of the wall densities catches up once per iteration
     wall_scales_on_device_ = false;
-    std::vector<double> sc(wall_faces_.size());
-    backend_->wall_inflow_scales(wall_faces_.data(), wall_faces_.size(), sc.data());
-    for (std::size_t i = 0; i < wall_faces_.size(); ++i) inflow_[static_cast<std::size_t>(wall_faces_[i])].scale = sc[i];
+    std::vector<double> scales(wall_faces_.size());
+    backend_->wall_inflow_scales(wall_faces_.data(), wall_faces_.size(), scales.data());
+    for (std::size_t i = 0; i < wall_faces_.size(); ++i)
+      bnd_.inflow[static_cast<std::size_t>(wall_faces_[i])].scale = scales[i];
   }
+  // residual norm of the first inner sweep: sqrt(w sum_c (sum R^2)_c / V_c / sum_c V_c)
   double sum = 0.0, vol = 0.0;
   for (Index c = 0; c < s_.n_cells(); ++c) {
     const double V = s_.volume[static_cast<std::size_t>(c)];
@@ -586,9 +304,9 @@ void KineticSolver::boundary_fluxes(double* fp, double* fm, double* fs, std::siz
   for (std::size_t b = 0; b < nb; ++b) {
     const auto o = static_cast<std::size_t>(s_.bface_new_to_old[b]);
     for (std::size_t k = 0; k < 5; ++k) {
-      fp[o * 5 + k] = F_plus_[b][k];
-      fm[o * 5 + k] = F_minus_[b][k];
-      fs[o * 5 + k] = face_state_[b][k];
+      fp[o * 5 + k] = bnd_.flux_out[b][k];
+      fm[o * 5 + k] = bnd_.flux_in[b][k];
+      fs[o * 5 + k] = bnd_.face_state[b][k];
     }
   }
 }
@@ -673,8 +391,8 @@ void KineticSolver::set_mu0(double mu0) {
 }
 
 void KineticSolver::set_accommodation(int patch, double alpha) {
-  GSK_CHECK(patch >= 0 && patch < static_cast<int>(patches_.size()), "patch index out of range");
-  PatchCondition& pc = patches_[static_cast<std::size_t>(patch)];
+  GSK_CHECK(patch >= 0 && patch < static_cast<int>(bnd_.patches().size()), "patch index out of range");
+  PatchCondition& pc = bnd_.patches()[static_cast<std::size_t>(patch)];
   GSK_CHECK(pc.kind == kBcMaxwellWall, "accommodation can be changed on Maxwell walls only");
   GSK_CHECK(alpha > 0.0, "accommodation must be positive");
   pc.accommodation = alpha;

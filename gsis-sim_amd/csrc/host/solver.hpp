@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "backend.hpp"
+#include "boundary.hpp"
 #include "setup.hpp"
 
 namespace gsk {
@@ -43,7 +44,7 @@ class KineticSolver {
   void update_mass_flow_pressure();
   const gsk_iteration_info& sweep(bool update_pressure);
   long long iteration() const { return iteration_; }
-  const std::vector<double>& patch_mass_flow() const { return patch_mass_flow_; }
+  const std::vector<double>& patch_mass_flow() const { return bnd_.patch_flow; }
 
   void moments(double* out, std::size_t n);  // original numbering
   void boundary_fluxes(double* fp, double* fm, double* fs, std::size_t nb);
@@ -76,50 +77,42 @@ class KineticSolver {
   void set_boundary_iterations(long long n);
 
  private:
-  void correct_inflow(gsk_inflow_state& in) const;
-  void init_static_inflow();
   void reset_after_initialization();
-  void init_wall_scale_from_cells();
-  double cell_value(const std::vector<double>& m, Index c, int k) const {
-    return m[static_cast<std::size_t>(c) * 15 + static_cast<std::size_t>(k)];
-  }
-  void update_pressure_inflow();
-  std::array<double, 10> compute_wall_unit_sums(Index b) const;
-  void process_boundary_sums();
-  void compute_epsilon();
+  // prepare, in stages
+  void read_moments();
+  void rescale_closed_domain();
+  void iteration_statistics();
+  void boundary_rounds();
+  void measure_convergence();  // eq. (59) epsilon against the previous iteration's moments
   void refresh_wall_inflow();
+  void correct_pressure_states(const std::vector<Index>& faces);
 
   const KineticSetup& s_;
-  std::vector<PatchCondition> patches_;  // mutable copy (mass-flow inlets adjust p)
   std::unique_ptr<Backend> backend_;
+  BoundaryModel bnd_;
   long long iteration_ = 0, version_ = 0, prepared_version_ = -1;
   double mu0_ = 0.0;
   double inflow_tol_ = 1e-10;
   long long boundary_iterations_ = 1;
   std::vector<double> specular_;  // 1 - alpha per boundary face (Maxwell walls)
   gsk_iteration_info info_{};
-  std::vector<double> patch_mass_flow_;
-  std::vector<double> moments_, prev_moments_;
+  std::vector<double> moments_, prev_moments_;  // 15 per cell, internal numbering
   bool has_prev_ = false;
   std::vector<double> bsums_;
-  std::vector<gsk_inflow_state> inflow_;
-  std::vector<std::array<double, 10>> wall_unit_moments_;
-  std::vector<std::array<double, 5>> F_plus_, F_minus_, face_state_;
-  bool has_face_state_ = false;
   std::vector<double> residual_cell_, inner_residual_;
   double mass_target_ = 0.0;
-  gsk_inflow_state wall_cache_{};
+  bool has_mass_target_ = false;
+  // wall faces and, per wall face, whether it is a Maxwell wall and its unit-density incoming mass
+  // flux (the inner-sweep wall refresh)
   std::vector<int> wall_faces_, wall_maxwell_;
+  std::vector<double> wall_unit0_;
+  bool wall_scales_on_device_ = false;  // the wall scales in bnd_.inflow are behind the backend's
   // section cuts of the last requested (axis, positions) and, per cut face, its owner and CSR entry
   int section_axis_ = -1;
   std::vector<double> section_positions_;
   std::vector<SectionCut> section_cuts_;
   std::vector<int> section_cells_, section_entries_;
   void build_sections(int axis, const std::vector<double>& positions);
-  std::vector<double> wall_unit0_;
-  bool wall_scales_on_device_ = false;  // inflow_ scales of wall faces are behind the backend's
-  int wall_cache_patch_ = -1;
-  bool has_mass_target_ = false;
 };
 
 }  // namespace gsk
