@@ -200,3 +200,23 @@ def kinetic_case(mg, directory, mesh, bcs, Kn=0.5, n_vel=10, vmax=5.0, kinetic=N
     ref = g.Setup(path).describe()["reference"]
     write(ref["pressure_Pa"], ref["velocity_m_s"])
     return path
+
+
+def tube_coarse_case(mg, directory, delta, kinetic=None, name=None):
+    """The reference's coarse tube-to-vacuum case (cases/tube_vacuum/kinetic/tube_coarse_delta{1,10}_cis.toml)
+    on its mesh (tests/golden/tube_hex_coarse.npz, written back as a .pmesh): helium, L_ref = R = 1 mm,
+    T0 = 273.15 K, pressure inlet at (p_in, T0), vacuum outlet, diffuse wall at T0, 30 x 24 x 24
+    velocities on [-10, 10] x [-8, 8]^2, sweep_blocks = 0 (automatic), the given [kinetic] settings."""
+    import json
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tube_hex_coarse.npz"))
+    mesh = dict(points=g["points"], face_nodes=g["face_nodes"], owner=g["owner"], neighbour=g["neighbour"],
+                n_cells=int(g["n_cells"]), patches=json.loads(str(g["patches"])))
+    mg.write_pmesh(os.path.join(directory, "tube_hex_coarse.pmesh"), mesh, {"generator": "tests/golden/tube_hex_coarse.npz"})
+    name = name or f"tube_coarse_delta{delta:g}_cis"
+    path = os.path.join(directory, name + ".toml")
+    bnd = [dict(patch="inlet", type="pressure_inlet", temperature=T0), dict(patch="outlet", type="vacuum"),
+           dict(patch="wall", type="diffuse_wall", temperature=T0)]
+    mg.write_case(path, "tube_hex_coarse.pmesh", name, bnd, [30, 24, 24], [(-10.0, 10.0), (-8.0, 8.0), (-8.0, 8.0)],
+                  dict(L_ref=1.0e-3, T0=T0, delta_rp=float(delta)), dict({"sweep_blocks": 0}, **(kinetic or {})))
+    return path
