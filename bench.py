@@ -318,7 +318,7 @@ def main():
     ap.add_argument("--fd-check", action="store_true", help="with --sensitivity: central differences (4 solves)")
     ap.add_argument("--fd-residual", type=float, default=1e-10, help="residual target of the difference solves")
     ap.add_argument("--polish", type=int, default=120, help="GMRES iterations polishing the BiCGStab solution")
-    ap.add_argument("--sens-eps", type=float, default=1e-7,
+    ap.add_argument("--sens-eps", type=float, default=1e-5,
                     help="relative perturbation of the difference Jacobian-vector products")
     ap.add_argument("--fd-params", default="delta_rp,alpha", help="parameters the differences check")
     ap.add_argument("--fd-step", type=float, default=1e-3,

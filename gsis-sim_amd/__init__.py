@@ -122,7 +122,7 @@ EXPORTS = [
     "gsk_solver_residual_per_cell", "gsk_solver_section_mass_flow", "gsk_solver_section_cuts", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
     "gsk_solver_set_distribution_device", "gsk_solver_residual_device", "gsk_solver_precondition_device",
     "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
-    "gsk_solver_set_boundary_iterations",
+    "gsk_solver_set_boundary_iterations", "gsk_solver_freeze_limiters",
 ]
 
 # fixed device scratch of gsk_backend_gsis_correction_noalloc (GSK_NOALLOC_SCRATCH_BYTES, GSK_NOALLOC_MAX_CELLS)
@@ -484,6 +484,11 @@ class Solver:
 
     def set_boundary_iterations(self, n: int) -> None:
         _check(lib().gsk_solver_set_boundary_iterations(self._h, C.c_int64(n)))
+
+    def freeze_limiters(self, on: bool) -> None:
+        """Record every limiter branch at the next residual_device and follow it afterwards (on), or
+        restore the normal limiter (off)."""
+        _check(lib().gsk_solver_freeze_limiters(self._h, C.c_int32(1 if on else 0)))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and _lib_handle is not None:

@@ -713,6 +713,13 @@ int gsk_solver_set_newton_tolerances(gsk_solver* sv, double cell_tol, double inf
   });
 }
 
+int gsk_solver_freeze_limiters(gsk_solver* sv, int32_t on) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->freeze_limiters(on != 0);
+  });
+}
+
 int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n) {
   return guarded([&] {
     need(sv, "solver");

@@ -94,6 +94,9 @@ class Backend {
   // precision, and 1e-10)
   virtual void set_newton_tolerances(double cell_tol, double inflow_tol) { unsupported(); }
   virtual void set_specular_fraction(const double* frac, std::size_t n) { unsupported(); }  // 1 - alpha per face
+  // limiter branches of every limited gradient: 0 off, 1 record at the next residual evaluation and
+  // follow the record afterwards, 2 follow the last record
+  virtual void set_limiter_branches(int mode) { unsupported(); }
 
  private:
   [[noreturn]] void unsupported() const {

@@ -162,25 +162,74 @@ std::array<double, 5> BoundaryModel::reconstruct_face_state(Index b, const std::
   const Vec3 to_face = s_.geometry.face_center[face] - x_own;
   const double K = s_.params.limiter_k;
   const double eps2 = K * K * K * s_.volume[static_cast<std::size_t>(owner)];
+  const bool record = branches_ == Branches::Record, follow = branches_ == Branches::Follow;
+  FaceBranch* fb = (record || follow) ? &face_branch(b) : nullptr;
   W out{};
   for (std::size_t q = 0; q < 5; ++q) {
     Vec3 grad{};
     double hi = own[q], lo = own[q];
+    int hi_slot = -1, lo_slot = -1;
     for (std::size_t j = 0; j < nb_state.size(); ++j) {
       grad += (nb_state[j][q] - own[q]) * s_.lsq[static_cast<std::size_t>(k0) + j];
-      hi = std::max(hi, nb_state[j][q]);
-      lo = std::min(lo, nb_state[j][q]);
+      if (hi < nb_state[j][q]) {
+        hi = nb_state[j][q];
+        hi_slot = static_cast<int>(j);
+      }
+      if (nb_state[j][q] < lo) {
+        lo = nb_state[j][q];
+        lo_slot = static_cast<int>(j);
+      }
     }
+    auto face_offset = [&](std::size_t j) {
+      return s_.geometry.face_center[static_cast<std::size_t>(s_.csr.faces[static_cast<std::size_t>(k0) + j])] - x_own;
+    };
     double phi = 1.0;
-    if (s_.limiter_mode != 0)
+    if (follow) {
+      hi = fb->hi_slot[q] < 0 ? own[q] : nb_state[static_cast<std::size_t>(fb->hi_slot[q])][q];
+      lo = fb->lo_slot[q] < 0 ? own[q] : nb_state[static_cast<std::size_t>(fb->lo_slot[q])][q];
+      const int j = fb->phi_slot[q];
+      if (j >= 0)
+        phi = dev::venkat_frozen<double>(dot(grad, face_offset(static_cast<std::size_t>(j))),
+                                         fb->positive_side[q] ? hi - own[q] : lo - own[q], eps2, fb->clamp[q]);
+    } else if (s_.limiter_mode != 0) {
+      int phi_slot = -1;
+      unsigned phi_clamp = 0;
+      bool phi_positive = false;
       for (std::size_t j = 0; j < nb_state.size(); ++j) {
         if (periodic[j]) continue;
-        const Vec3& xf = s_.geometry.face_center[static_cast<std::size_t>(s_.csr.faces[static_cast<std::size_t>(k0) + j])];
-        phi = std::min(phi, dev::venkat_face<double>(dot(grad, xf - x_own), hi - own[q], lo - own[q], eps2));
+        const double d2 = dot(grad, face_offset(j));
+        unsigned clamp = 0;
+        const double v = dev::venkat_tracked<double>(d2, hi - own[q], lo - own[q], eps2, clamp);
+        if (v < phi) {  // std::min(phi, v) keeps phi on ties
+          phi = v;
+          phi_slot = static_cast<int>(j);
+          phi_clamp = clamp;
+          phi_positive = d2 > 0.0;
+        }
       }
+      if (record) {
+        fb->phi_slot[q] = phi_slot;
+        fb->clamp[q] = phi_clamp;
+        fb->positive_side[q] = phi_positive;
+        fb->hi_slot[q] = hi_slot;
+        fb->lo_slot[q] = lo_slot;
+      }
+    } else if (record) {
+      fb->phi_slot[q] = -1;
+      fb->hi_slot[q] = hi_slot;
+      fb->lo_slot[q] = lo_slot;
+    }
     out[q] = own[q] + phi * dot(grad, to_face);
   }
-  return out[0] > 0.0 && out[4] > 0.0 ? out : own;
+  bool keep_owner = !(out[0] > 0.0 && out[4] > 0.0);
+  if (record) fb->owner_state = keep_owner;
+  if (follow) keep_owner = fb->owner_state;
+  return keep_owner ? own : out;
+}
+
+BoundaryModel::FaceBranch& BoundaryModel::face_branch(Index b) const {
+  if (branch_.size() != static_cast<std::size_t>(s_.n_bfaces())) branch_.assign(static_cast<std::size_t>(s_.n_bfaces()), {});
+  return branch_[static_cast<std::size_t>(b)];
 }
 
 // Pressure faces (inlets, and outlets with p > 0): from the reconstructed interior state (p_i, u_i,

@@ -44,6 +44,11 @@ class BoundaryModel {
   void absorb(const std::vector<double>& sums);
   // Wall densities only, from the sums of the wall faces (the refresh between inner sweeps).
   void refresh_walls(const std::vector<double>& sums, const std::vector<int>& wall_faces);
+  // Branches of the pressure-face reconstruction's limiter (see reconstruct_face_state): Off, Record
+  // (compute normally and keep every choice) or Follow (take the recorded choices), for
+  // sensitivity solves that differentiate one smooth branch.
+  enum class Branches { Off, Record, Follow };
+  void set_branches(Branches b) { branches_ = b; }
   void forget_face_states() { has_face_state_ = false; }
   bool has_face_states() const { return has_face_state_; }
 
@@ -61,6 +66,18 @@ class BoundaryModel {
   const KineticSetup& s_;
   std::vector<PatchCondition> patches_;  // mutable copy: mass-flow inlets adjust p, Maxwell walls alpha
   bool has_face_state_ = false;
+  // per boundary face and variable: neighbour (CSR slot) whose limiter value set phi (-1: phi = 1),
+  // its clamp and the sign of its projection, the slots of the largest and smallest neighbour
+  // value (-1: the owner's), and whether the positivity fallback kept the owner's state
+  struct FaceBranch {
+    int phi_slot[5], hi_slot[5], lo_slot[5];
+    unsigned clamp[5];
+    bool positive_side[5];
+    bool owner_state;
+  };
+  Branches branches_ = Branches::Off;
+  mutable std::vector<FaceBranch> branch_;
+  FaceBranch& face_branch(Index b) const;
 };
 
 }  // namespace gsk

@@ -374,6 +374,17 @@ void KineticSolver::residual_device(void* out, std::size_t n) {
   ++version_;
   prepare();
   backend_->residual_field_device(out);
+  if (freeze_ == Freeze::Record) {  // the backend follows its record from now on, and so does the host
+    freeze_ = Freeze::Follow;
+    bnd_.set_branches(BoundaryModel::Branches::Follow);
+  }
+}
+
+void KineticSolver::freeze_limiters(bool on) {
+  backend_->set_limiter_branches(on ? 1 : 0);
+  freeze_ = on ? Freeze::Record : Freeze::Off;
+  bnd_.set_branches(on ? BoundaryModel::Branches::Record : BoundaryModel::Branches::Off);
+  ++version_;
 }
 
 void KineticSolver::precondition_device(const void* r, void* out, std::size_t n) {

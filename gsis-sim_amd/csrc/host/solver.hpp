@@ -75,6 +75,11 @@ class KineticSolver {
   // reconstruction reads the boundary face states of the previous round, so R(f) is a function of f
   // alone once these rounds have converged
   void set_boundary_iterations(long long n);
+  // Limiter branches for sensitivity solves: on = the next residual_device records the branch of
+  // every limiter (the kinetic limited gradients and the pressure-face reconstruction) at the
+  // current f, and later evaluations follow that record, so R is smooth around the recorded state;
+  // off = the normal limiter.
+  void freeze_limiters(bool on);
 
  private:
   void reset_after_initialization();
@@ -94,6 +99,7 @@ class KineticSolver {
   double mu0_ = 0.0;
   double inflow_tol_ = 1e-10;
   long long boundary_iterations_ = 1;
+  enum class Freeze { Off, Record, Follow } freeze_ = Freeze::Off;
   std::vector<double> specular_;  // 1 - alpha per boundary face (Maxwell walls)
   gsk_iteration_info info_{};
   std::vector<double> moments_, prev_moments_;  // 15 per cell, internal numbering

@@ -586,7 +586,7 @@ __device__ __forceinline__ bool xcd_tile(unsigned L, int n_cells, int tiles, int
   return true;
 }
 
-template <class F>
+template <class F, int Mode = 0>  // Mode: limiter branches (0 off, 1 record, 2 follow the record)
 __global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles, unsigned L0) {
   int c, tile;
   if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
@@ -596,7 +596,9 @@ __global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int
   F gx, gy, gz;
   const EllRow<F>& r = v.ell[c];
   const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p], fi = f_at(v, c, p);
-  if ((r.skip | r.bnd) == 0)
+  if constexpr (Mode != 0)
+    limited_gradient_row_on_branch<Mode>(v, r, c, p, x, y, z, fi, gx, gy, gz);
+  else if ((r.skip | r.bnd) == 0)
     limited_gradient_row<true>(v, r, c, p, x, y, z, fi, gx, gy, gz);
   else
     limited_gradient_row<false>(v, r, c, p, x, y, z, fi, gx, gy, gz);
@@ -1308,6 +1310,7 @@ class HipBackend final : public Backend {
     zd_.upload(s.xi_z, static_cast<std::size_t>(nv_), stream_);
     // mesh tables
     const int n_entries = s.csr_offsets[nc_];
+    for (int c = 0; c < nc_; ++c) max_faces_ = std::max(max_faces_, s.csr_offsets[c + 1] - s.csr_offsets[c]);
     csr_off_.upload(s.csr_offsets, static_cast<std::size_t>(nc_) + 1, stream_);
     std::vector<dev::CsrE<F>> csr(static_cast<std::size_t>(n_entries));
     for (int k = 0; k < n_entries; ++k) {
@@ -1743,8 +1746,15 @@ class HipBackend final : public Backend {
     const long long work = work_items(chunk_);
     const unsigned grid_irr = capped_grid(static_cast<long long>(n_irregular_) * tiles_);
     begin(kGradient);
-    for (long long L0 = 0; L0 < work; L0 += grid_cap_)
-      dev::k_gradient<F><<<capped_grid(work - L0), dev::kBlock, 0, stream_>>>(v, ch, tiles_, static_cast<unsigned>(L0));
+    for (long long L0 = 0; L0 < work; L0 += grid_cap_) {
+      const unsigned grid = capped_grid(work - L0), l0 = static_cast<unsigned>(L0);
+      if (v.lim_mode == 1)
+        dev::k_gradient<F, 1><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, l0);
+      else if (v.lim_mode == 2)
+        dev::k_gradient<F, 2><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, l0);
+      else
+        dev::k_gradient<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, l0);
+    }
     if (n_irregular_ > 0)
       dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
     HIPCHK(hipGetLastError());
@@ -1849,6 +1859,20 @@ class HipBackend final : public Backend {
       copy_chunk(o + k0, static_cast<std::size_t>(nv_), rres_.p, static_cast<std::size_t>(chunk_), ch.kc);
     }
     HIPCHK(hipStreamSynchronize(stream_));
+    if (lim_mode_ == 1) lim_mode_ = 2;  // recorded at this state: later evaluations follow the record
+  }
+
+  void set_limiter_branches(int mode) override {
+    GSK_CHECK(mode >= 0 && mode <= 2, "limiter branch mode must be 0 (off), 1 (record) or 2 (follow)");
+    if (mode != 0) {
+      GSK_CHECK(max_faces_ <= dev::kLimMaxFaces, "limiter branches hold at most 15 faces per cell");
+      if (branch_.n == 0) {
+        branch_.alloc(static_cast<std::size_t>(nc_) * static_cast<std::size_t>(nv_));
+        HIPCHK(hipMemsetAsync(branch_.p, 0, branch_.bytes(), stream_));
+      }
+      GSK_CHECK(mode == 1 || lim_mode_ != 0, "no limiter branches have been recorded");
+    }
+    lim_mode_ = mode;
   }
 
   void precondition_device(const void* r, void* out) override {
@@ -2230,6 +2254,8 @@ class HipBackend final : public Backend {
     v.discrete_eq = discrete_eq_;
     v.deq_fallback = deq_fallback_;
     v.cold_bgk = cold_bgk_;
+    v.lim_mode = lim_mode_;
+    v.lim_branch = branch_.p;
     return v;
   }
 
@@ -2240,6 +2266,8 @@ class HipBackend final : public Backend {
   int grid_n_[3] = {0, 0, 0};
   double mu0_ = 0, omega_ = 0, Pr_ = 0, cold_bgk_ = 0, diag_scale_ = 1, interface_damping_ = 0, weight_ = 0;
   int discrete_eq_ = 0, deq_fallback_ = 0, limiter_mode_ = 1, positivity_ = 0;
+  int lim_mode_ = 0, max_faces_ = 0;  // limiter branches (set_limiter_branches); most faces of a cell
+  DBuf<unsigned> branch_;
   dev::DeqGrid deq_{};
   DBuf<F> xi_x_, xi_y_, xi_z_;
   DBuf<double> xd_, yd_, zd_;

@@ -174,6 +174,41 @@ GSK_HD F venkat_face(F d2, F dmaxv, F dminv, F eps2) {
   return r < F(0) ? F(0) : (F(1) < r ? F(1) : r);
 }
 
+// The same with the branch it took (clamp: 0 none, 1 to 0 (also a non-positive denominator), 2 to 1
+// (also d2 = 0)), and its value on a given branch with d1 already chosen by the sign of d2 (the
+// limiter-branch evaluations of sensitivity solves).
+template <class F>
+GSK_HD F venkat_tracked(F d2, F dmaxv, F dminv, F eps2, unsigned& clamp) {
+  F d1;
+  if (d2 > F(0)) {
+    d1 = dmaxv;
+  } else if (d2 < F(0)) {
+    d1 = dminv;
+  } else {
+    clamp = 2;
+    return F(1);
+  }
+  const F num = (d1 * d1 + eps2) + F(2) * d1 * d2;
+  const F den = d1 * d1 + F(2) * d2 * d2 + d1 * d2 + eps2;
+  clamp = 1;
+  if (!(den > F(0))) return F(0);
+  const F r = num / den;
+  if (r < F(0)) return F(0);
+  clamp = 2;
+  if (F(1) < r) return F(1);
+  clamp = 0;
+  return r;
+}
+
+template <class F>
+GSK_HD F venkat_frozen(F d2, F d1, F eps2, unsigned clamp) {
+  if (clamp == 1) return F(0);
+  if (clamp == 2) return F(1);
+  const F num = (d1 * d1 + eps2) + F(2) * d1 * d2;
+  const F den = d1 * d1 + F(2) * d2 * d2 + d1 * d2 + eps2;
+  return num / den;
+}
+
 // ---- moments about the per-cell shift u0 ----
 template <class F>
 GSK_HD void moment_terms(F f, F cx, F cy, F cz, F (&t)[kMomentSums]) {

@@ -88,6 +88,10 @@ struct View {
   DeqGrid deq;
   int discrete_eq, deq_fallback;
   double cold_bgk;
+  // limiter branches of sensitivity solves (lim_mode 0: off): 1 records, per (cell, velocity), the
+  // branch every limited gradient takes; 2 evaluates every limited gradient on its recorded branch
+  int lim_mode;
+  unsigned* lim_branch;  // [cell][velocity]
 };
 
 // velocity chunk workspace [n_cells][stride], position t = p - k0
@@ -181,6 +185,149 @@ __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, cons
   limit_from_extremes(v, cell, fi, fmx, fmn, proj_max, proj_min, gx, gy, gz, ogx, ogy, ogz);
 }
 
+// ---- Limiter branches (sensitivity solves) ----
+// The limited gradient is piecewise smooth in f: the limiter takes the largest and the most negative
+// face projection, the largest and smallest face value, the larger of |f_max| and |f_min| (relative
+// eps), the smaller of two limiter values, and clamps each to [0, 1]. A sensitivity solve needs
+// R(f) differentiated on the branch the converged state sits on; crossing a branch inside a
+// difference product makes it nonlinear in the direction. So at the converged state every choice is
+// recorded (Mode 1), and later evaluations (Mode 2) take the recorded candidates and clamps, which
+// makes R smooth around that state; differences of it then give the branch's Jacobian products to
+// second order in the step. Record layout (one 32-bit word per (cell, velocity)):
+//   [0,4) face of the largest projection   [4,8) face of the most negative one   (kNoCandidate: stayed 0)
+//   [8,12) face of f_max   [12,16) face of f_min   (kNoCandidate: the cell's own value)
+//   16 eps from |f_max| (else |f_min|)   [17,19) max-side clamp   [19,21) min-side clamp
+//   (0 none, 1 to 0, 2 to 1)   21 phi from the min side   [22,24) positivity (0 none, 1 phi =
+//   f_i / -proj_min, 2 no gradient)
+constexpr unsigned kNoCandidate = 15;
+constexpr int kLimMaxFaces = 15;
+
+// phi * grad for one (cell, velocity) from its n face values fk (faces whose bit is set in `use`
+// count for the extremes) and face offsets ds, recording (Mode 1) or following (Mode 2) the branch.
+template <int Mode, class F>
+__device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, const F* fk, unsigned use,
+                                const F* dsx, const F* dsy, const F* dsz, F gx, F gy, F gz, F& ogx, F& ogy,
+                                F& ogz) {
+  unsigned* slot = v.lim_branch + static_cast<i64>(cell) * v.nv + p;
+  if (v.limiter_mode == kLimNone && v.positivity == 0) {
+    ogx = gx;
+    ogy = gy;
+    ogz = gz;
+    return;
+  }
+  const unsigned code = Mode == 2 ? *slot : 0u;
+  if (v.positivity != 0 && (Mode == 2 ? ((code >> 22) & 3u) == 2u : !(fi > F(0)))) {
+    ogx = ogy = ogz = F(0);
+    if (Mode == 1) *slot = 2u << 22;
+    return;
+  }
+  unsigned kmx = kNoCandidate, kmn = kNoCandidate, imx = kNoCandidate, imn = kNoCandidate;
+  F fmx = fi, fmn = fi, proj_max = F(0), proj_min = F(0);
+  if (Mode == 1) {
+    for (int k = 0; k < n; ++k) {
+      if (!((use >> k) & 1u)) continue;
+      if (fk[k] > fmx) {
+        fmx = fk[k];
+        imx = static_cast<unsigned>(k);
+      }
+      if (fk[k] < fmn) {
+        fmn = fk[k];
+        imn = static_cast<unsigned>(k);
+      }
+    }
+    for (int k = 0; k < n; ++k) {
+      if (!((use >> k) & 1u)) continue;
+      const F proj = dot3(gx, gy, gz, dsx[k], dsy[k], dsz[k]);
+      if (proj > proj_max) {
+        proj_max = proj;
+        kmx = static_cast<unsigned>(k);
+      }
+      if (proj < proj_min) {
+        proj_min = proj;
+        kmn = static_cast<unsigned>(k);
+      }
+    }
+  } else {
+    kmx = code & 15u;
+    kmn = (code >> 4) & 15u;
+    imx = (code >> 8) & 15u;
+    imn = (code >> 12) & 15u;
+    if (imx != kNoCandidate) fmx = fk[imx];
+    if (imn != kNoCandidate) fmn = fk[imn];
+    if (kmx != kNoCandidate) proj_max = dot3(gx, gy, gz, dsx[kmx], dsy[kmx], dsz[kmx]);
+    if (kmn != kNoCandidate) proj_min = dot3(gx, gy, gz, dsx[kmn], dsy[kmn], dsz[kmn]);
+  }
+  F eps2 = v.eps2[cell];
+  bool eps_from_max = false;
+  if (v.limiter_mode == kLimRel) {
+    eps_from_max = Mode == 2 ? ((code >> 16) & 1u) != 0u : !(dabs(fmx) < dabs(fmn));
+    const F s = eps_from_max ? dabs(fmx) : dabs(fmn);
+    eps2 *= s * s;
+  }
+  const F dmx = fmx - fi, dmn = fmn - fi;
+  F phi = F(1);
+  unsigned cmx = 2, cmn = 2, side = 0, pos = 0;
+  if (v.limiter_mode != kLimNone) {
+    F vmx, vmn;
+    if (Mode == 1) {
+      vmx = venkat_tracked(proj_max, dmx, dmn, eps2, cmx);
+      vmn = venkat_tracked(proj_min, dmx, dmn, eps2, cmn);
+      side = vmn < vmx ? 1u : 0u;
+    } else {
+      cmx = (code >> 17) & 3u;
+      cmn = (code >> 19) & 3u;
+      side = (code >> 21) & 1u;
+      vmx = kmx == kNoCandidate ? F(1) : venkat_frozen(proj_max, dmx, eps2, cmx);
+      vmn = kmn == kNoCandidate ? F(1) : venkat_frozen(proj_min, dmn, eps2, cmn);
+    }
+    phi = side ? vmn : vmx;
+  }
+  if (v.positivity != 0) {
+    if (Mode == 1) {
+      if (proj_min < F(0)) {
+        const F cap = fi / (-proj_min);
+        if (cap < phi) {
+          phi = cap;
+          pos = 1;
+        }
+      }
+    } else if (((code >> 22) & 3u) == 1u) {
+      phi = fi / (-proj_min);
+    }
+  }
+  if (Mode == 1)
+    *slot = kmx | (kmn << 4) | (imx << 8) | (imn << 12) | ((eps_from_max ? 1u : 0u) << 16) | (cmx << 17) |
+            (cmn << 19) | (side << 21) | (pos << 22);
+  ogx = phi * gx;
+  ogy = phi * gy;
+  ogz = phi * gz;
+}
+
+// The limited gradient from the CSR entries (any face count up to kLimMaxFaces), recording or
+// following the branch.
+template <int Mode, class F>
+__device__ void limited_gradient_on_branch(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy,
+                                           F& ogz) {
+  const int e0 = v.csr_off[cell], n = v.csr_off[cell + 1] - e0;
+  F fk[kLimMaxFaces], dsx[kLimMaxFaces], dsy[kLimMaxFaces], dsz[kLimMaxFaces];
+  unsigned use = 0;
+  F gx = F(0), gy = F(0), gz = F(0);
+  for (int k = 0; k < n && k < kLimMaxFaces; ++k) {
+    const CsrE<F>& e = v.csr[e0 + k];
+    fk[k] = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
+    const F d = fk[k] - fi;
+    gx += e.cx * d;
+    gy += e.cy * d;
+    gz += e.cz * d;
+    dsx[k] = e.dsx;
+    dsy[k] = e.dsy;
+    dsz[k] = e.dsz;
+    if (!(e.other < 0 && v.bf[e.bface].kind == kBcPeriodic)) use |= 1u << k;
+  }
+  limit_on_branch<Mode>(v, cell, p, fi, n < kLimMaxFaces ? n : kLimMaxFaces, fk, use, dsx, dsy, dsz, gx, gy, gz, ogx,
+                        ogy, ogz);
+}
+
 // gradient from at most kGather entries ents[0..n): neighbour values gathered first, then combined
 // in entry order
 template <class F>
@@ -212,6 +359,14 @@ __device__ __forceinline__ void gradient_gathered(const View<F>& v, int cell, in
 
 template <class F>
 __device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
+  if (v.lim_mode == 1) {
+    limited_gradient_on_branch<1>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
+    return;
+  }
+  if (v.lim_mode == 2) {
+    limited_gradient_on_branch<2>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
+    return;
+  }
   const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
   if (e1 - e0 <= kGather) {
     gradient_gathered(v, cell, p, x, y, z, fi, v.csr + e0, e1 - e0, ogx, ogy, ogz);
@@ -280,6 +435,29 @@ __device__ __forceinline__ void limited_gradient_row(const View<F>& v, const Ell
     proj_min = nmin(proj_min, proj);
   }
   limit_from_extremes(v, cell, fi, fmx, fmn, proj_max, proj_min, gx, gy, gz, ogx, ogy, ogz);
+}
+
+// The row form of limited_gradient_on_branch (the gradient kernel's path).
+template <int Mode, class F>
+__device__ void limited_gradient_row_on_branch(const View<F>& v, const EllRow<F>& r, int cell, int p, F x, F y, F z,
+                                               F fi, F& ogx, F& ogy, F& ogz) {
+  const F* fp = v.f + p;
+  F fk[kGather];
+  unsigned use = 0;
+  F gx = F(0), gy = F(0), gz = F(0);
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) fk[k] = fp[r.nbf[k]];
+#pragma unroll
+  for (int k = 0; k < kGather; ++k) {
+    if ((r.skip >> k) & 1u) continue;
+    if ((r.bnd >> k) & 1u) fk[k] = boundary_value_first_order(v, r.bface[k], r.sx[k], r.sy[k], r.sz[k], cell, p, x, y, z, fi);
+    const F d = fk[k] - fi;
+    gx = madd(gx, r.cx[k], d);
+    gy = madd(gy, r.cy[k], d);
+    gz = madd(gz, r.cz[k], d);
+    use |= 1u << k;
+  }
+  limit_on_branch<Mode>(v, cell, p, fi, kGather, fk, use, r.dsx, r.dsy, r.dsz, gx, gy, gz, ogx, ogy, ogz);
 }
 
 // outgoing reconstructed value at a boundary face (kinetic_kernels.hpp:204-211)

@@ -6,23 +6,25 @@ At the converged state f* the steady residual vanishes, R(f*; p) = 0, so
     dMdot/dp = dMdot/dp|_f + (dMdot/df) s,    J s = -dR/dp,    J = dR/df at f*.
 R is the solver's full steady residual (transport with the limited reconstruction, Shakhov
 collision with the discrete equilibrium, every boundary condition, wall densities and inflow states
-recomputed from f), evaluated on the device through gsk_solver_residual_device. J s = -dR/dp is
-solved by BiCGStab (then restarted GMRES on the true residual) with right preconditioning by the
-LU-SGS passes of one inner sweep (gsk_solver_precondition_device, the same level-scheduled kernels
-as the forward solve), with Jacobian-vector products as central FP64 differences of R. The step is a
-relative eps = 1e-7 of |f*|: R is only piecewise smooth (the limiter's min / max and clamp), and a
-larger step (1e-5 measured) crosses enough kinks that the difference products stop being linear in
-the direction, which leaves the true linear residual at 5-11% and the sensitivities off by 1-2e-4;
-at 1e-7 the rounding noise of the differences is still ~1e-9 of the signal. The linear system is
-ill-conditioned (as slow as the CIS iteration itself), so the products must be accurate to ~1e-9,
-and R must be a function of f alone. Two settings make it so
-for the duration of the sensitivity solve (restored afterwards): the discrete-equilibrium Newton
-solves inside R (cells and inflow states) run until the Newton step stops improving instead of
-stopping at a tolerance, and every prepare runs 10 inflow / boundary-sum rounds, because the
-pressure-inlet reconstruction reads the boundary face states of the previous round (one round per
-prepare in the forward solve, where the rounds converge along with the CIS iteration). With these,
-R(f*) repeats to ~1e-9 of a 1e-6 perturbation's response. One linear solve per parameter: for the two
-parameters here this costs what an adjoint solve would.
+recomputed from f), evaluated on the device through gsk_solver_residual_device. This is a forward
+(tangent) solve per parameter; with two parameters it costs what one adjoint solve and an extra
+right-hand side would, and it needs no transposed operator. J s = -dR/dp is solved by BiCGStab (then
+restarted GMRES on the true residual) with right preconditioning by the LU-SGS passes of one inner
+sweep (gsk_solver_precondition_device, the level-scheduled kernels of the forward solve).
+
+Jacobian-vector products. R is only piecewise smooth in f: every limited gradient takes the largest
+and the most negative face projection, the extreme neighbour values and the smaller of two limiter
+values, clamped to [0, 1], and the pressure-inlet reconstruction has its own limiter. A difference
+product that crosses one of those branches is not linear in the direction (with differences of the
+plain residual the true linear residual of the Krylov solve stayed at 1-4% and the sensitivities
+depended on the step). So at f* every branch is recorded (gsk_solver_freeze_limiters), and every
+evaluation of the solve follows the record: R is then smooth around f*, equal to R on f*'s branch,
+and central differences J z = (R(f* + h z) - R(f* - h z)) / 2h give that branch's Jacobian products to
+O(h^2) (h = 1e-5 |f*| / |z| by default; the rounding part is ~1e-11 of the product). The solve
+further sets the discrete-equilibrium Newton tolerances to 0 (iterate until the step stops improving,
+so R does not jump with the iteration count) and runs 10 inflow / boundary-sum rounds per prepare
+(the pressure-inlet state depends on the previous round's face states), and restores the case's
+settings afterwards.
 
 delta_rp enters only through the nondimensional viscosity scale mu0 = 1 / (sqrt(2) delta_rp); alpha
 only through the specular fraction 1 - alpha of the Maxwell-wall faces.
@@ -45,7 +47,7 @@ class SteadyProblem:
     parameter (None if alpha is not needed)."""
 
     def __init__(self, solver, inlet: int, mdot_factor: float, wall_patch: int | None = None,
-                 delta_rp: float | None = None, alpha: float | None = None, eps: float = 1e-7):
+                 delta_rp: float | None = None, alpha: float | None = None, eps: float = 1e-5):
         self.sv = solver
         self.inlet = inlet
         self.factor = mdot_factor
@@ -119,7 +121,7 @@ class SteadyProblem:
         self.f0_norm = float(torch.linalg.vector_norm(self.f0))
 
     def _jv(self, z: torch.Tensor) -> tuple[torch.Tensor, float]:
-        """J z and (dMdot/df) z by central differences of R and Mdot at f*."""
+        """J z and (dMdot/df) z by central differences of R and Mdot at f*, on f*'s limiter branches."""
         zn = float(torch.linalg.vector_norm(z))
         if zn == 0.0:
             return torch.zeros_like(z), 0.0
@@ -270,6 +272,7 @@ class SteadyProblem:
         self.sv.set_boundary_iterations(10)
         out = {}
         try:
+            self.sv.freeze_limiters(True)  # recorded by the residual evaluation at f* in _capture
             self._capture()
             for p in params:
                 dR, dM = self._param_derivative(p)
@@ -296,6 +299,7 @@ class SteadyProblem:
                 self.stats[f"{p}_krylov_relative_residual"] = rel
                 self.stats[f"{p}_true_relative_residual"] = true_rel
         finally:
+            self.sv.freeze_limiters(False)
             self.sv.set_newton_tolerances(*self.newton_tolerances)
             self.sv.set_boundary_iterations(self.boundary_iterations)
             if self.f0 is not None:
@@ -307,8 +311,8 @@ class SteadyProblem:
 
 class ConvergedMdot(torch.autograd.Function):
     """Mdot of the converged kinetic solve as a function of (delta_rp, alpha); backward returns the
-    sensitivities from SteadyProblem.sensitivities (difference Jacobian-vector products, LU-SGS right
-    preconditioning, BiCGStab with optional restarted-GMRES polishing)."""
+    sensitivities from SteadyProblem.sensitivities (branch-frozen difference Jacobian-vector products,
+    LU-SGS right preconditioning, BiCGStab with optional restarted-GMRES polishing)."""
 
     @staticmethod
     def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):

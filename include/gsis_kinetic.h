@@ -296,6 +296,12 @@ int gsk_solver_set_newton_tolerances(gsk_solver* sv, double cell_tol, double inf
 /* inflow / boundary-sum rounds per prepare ([kinetic].boundary_iterations); the pressure-inlet
  * reconstruction reads the face states of the previous round */
 int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n);
+/* on != 0: the next gsk_solver_residual_device records, at the current f, the branch every limiter
+ * takes (the limited gradients' extremes and clamps, and the pressure-face reconstruction's), and
+ * later evaluations follow that record, so that R is smooth around the recorded state and
+ * differences of it give the Jacobian products of that branch; on = 0 restores the normal limiter.
+ * Costs 4 bytes per cell and velocity of device memory while on. */
+int gsk_solver_freeze_limiters(gsk_solver* sv, int32_t on);
 gsk_backend* gsk_solver_backend(gsk_solver* sv);
 
 #ifdef __cplusplus
