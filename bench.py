@@ -240,37 +240,53 @@ def time_to_mdot(g, mg, args):
 
 def sensitivity_bench(g, mg, args):
     """Metric 3: cost of dMdot/d delta_rp and dMdot/d alpha (sensitivity solve) relative to one
-    converged forward solve, on the 40k-cell tube with a Maxwell wall (24^3, delta_rp = 10, FP64),
-    with central differences of re-converged solves as the check."""
+    converged forward solve, on the 40k-cell tube with a Maxwell wall (24^3, delta_rp = 10), with
+    central differences of re-converged FP64 solves as the check. --sens-forward mixed runs the
+    forward solve in the mixed form (FP32 f, matrix-core moment sums) and the sensitivity solve in
+    FP64 at that state; fp64 runs both in FP64 (the FP64 reference gradient)."""
     import gsis_sim_amd.sensitivity as sens
     import torch
 
     delta, alpha, h = 10.0, args.alpha, args.fd_step
     tube_args = WORKLOADS["tube40k_24cubed_fp64"][0]
+    mixed = args.sens_forward == "mixed"
     with tempfile.TemporaryDirectory() as d:
         # alpha = 1 through the Maxwell wall's smooth extension: the case gets a Maxwell wall (alpha < 1;
         # the reference turns alpha = 1 into a diffuse wall) and the solve then sets alpha, so central
         # differences may step past 1 (specular fraction 1 - alpha < 0)
-        path = mg.tube_case(d, n_vel=24, vrange=6.0, delta=delta, accommodation=min(alpha, 0.99),
-                            kinetic={"precision": "fp64"}, name="tube_maxwell", **tube_args)
-        setup = g.Setup(path)
+        def case(kinetic, name):
+            return mg.tube_case(d, n_vel=24, vrange=6.0, delta=delta, accommodation=min(alpha, 0.99), kinetic=kinetic,
+                                name=name, **tube_args)
+
+        setup = g.Setup(case({"precision": "fp64"}, "tube_maxwell"))
         sv = g.Solver(setup)
         prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
                                   alpha=alpha, eps=args.sens_eps)
         prob.log = lambda msg: log(msg)
+        sv32 = None
+        if mixed:
+            setup32 = g.Setup(case({"precision": "fp32", "moment_sums": "mfma"}, "tube_maxwell_mixed"))
+            sv32 = g.Solver(setup32)
+            sv32.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+            prob.use_forward_solver(sv32)
         d_t = torch.tensor(delta, dtype=torch.float64, requires_grad=True)
         a_t = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
         sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
-        log("forward solve")
-        mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=args.sens_residual)
+        log(f"forward solve ({args.sens_forward})")
+        target = args.mixed_residual if mixed else args.sens_residual
+        mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=target)
         fwd_s, fwd_its = prob.stats["forward_seconds"], prob.stats["forward_iterations"]
+        if sv32 is not None:
+            sv32.close()
         log(f"forward: {fwd_its} iterations, {fwd_s:.1f} s, Mdot {float(mdot.detach()):.10f}; sensitivity solve")
         prob.polish = args.polish
         mdot.backward()
-        out = {"metric": "sensitivity / forward cost", "workload": f"tube40k_24cubed_fp64_maxwell_alpha{alpha:g}",
+        out = {"metric": "sensitivity / forward cost",
+               "workload": f"tube40k_24cubed_maxwell_alpha{alpha:g}_{args.sens_forward}_forward",
+               "forward_form": "mixed (FP32 f, matrix-core moment sums)" if mixed else "fp64",
+               "sensitivity_form": "fp64, limiter branches frozen at the forward state",
                "delta_rp": delta, "alpha": alpha, "sens_eps": args.sens_eps, "mdot": float(mdot.detach()),
-               "forward_seconds": fwd_s,
-               "forward_iterations": fwd_its, "residual_target": args.sens_residual,
+               "forward_seconds": fwd_s, "forward_iterations": fwd_its, "forward_residual_target": target,
                "sensitivity_seconds": prob.stats["sensitivity_seconds"],
                "cost_ratio": prob.stats["sensitivity_seconds"] / fwd_s,
                "dmdot_ddelta": float(d_t.grad), "dmdot_dalpha": float(a_t.grad),
@@ -278,12 +294,14 @@ def sensitivity_bench(g, mg, args):
                           and not k.startswith("forward")}}
         log(f"sensitivities {out['dmdot_ddelta']:.8e} {out['dmdot_dalpha']:.8e} in {out['sensitivity_seconds']:.1f} s")
         if args.fd_check:
+            prob.use_forward_solver(None)
+
             def mdot_at(dl, al):
                 prob.set_parameters(dl, al)
                 m = prob.solve(residual_target=args.fd_residual, log=lambda msg: log(f"fd ({dl}, {al}): {msg}"))
                 log(f"fd ({dl}, {al}): Mdot {m:.12f} after {prob.stats['forward_iterations']} iterations")
                 return m
-            out.update({"fd_step": h, "fd_residual_target": args.fd_residual})
+            out.update({"fd_step": h, "fd_residual_target": args.fd_residual, "fd_form": "fp64"})
             if "delta_rp" in args.fd_params:
                 fd_d = (mdot_at(delta * (1 + h), alpha) - mdot_at(delta * (1 - h), alpha)) / (2 * delta * h)
                 out.update({"fd_dmdot_ddelta": fd_d, "rel_err_ddelta": abs(out["dmdot_ddelta"] - fd_d) / abs(fd_d)})
@@ -316,12 +334,16 @@ def main():
     ap.add_argument("--alpha", type=float, default=0.9, help="Maxwell-wall accommodation for --sensitivity")
     ap.add_argument("--sens-residual", type=float, default=1e-9, help="forward residual target for --sensitivity")
     ap.add_argument("--fd-check", action="store_true", help="with --sensitivity: central differences (4 solves)")
-    ap.add_argument("--fd-residual", type=float, default=1e-10, help="residual target of the difference solves")
+    ap.add_argument("--sens-forward", choices=["fp64", "mixed"], default="fp64",
+                    help="with --sensitivity: the form of the forward solve (the sensitivity solve is FP64)")
+    ap.add_argument("--mixed-residual", type=float, default=1e-6,
+                    help="forward residual target of the mixed form for --sens-forward mixed")
+    ap.add_argument("--fd-residual", type=float, default=1e-12, help="residual target of the difference solves")
     ap.add_argument("--polish", type=int, default=120, help="GMRES iterations polishing the BiCGStab solution")
     ap.add_argument("--sens-eps", type=float, default=1e-5,
                     help="relative perturbation of the difference Jacobian-vector products")
     ap.add_argument("--fd-params", default="delta_rp,alpha", help="parameters the differences check")
-    ap.add_argument("--fd-step", type=float, default=1e-3,
+    ap.add_argument("--fd-step", type=float, default=1e-4,
                     help="central-difference step (relative for delta_rp, absolute for alpha)")
     ap.add_argument("--residual-target", type=float, default=1e-6)
     ap.add_argument("--calibrate-copy", action="store_true",

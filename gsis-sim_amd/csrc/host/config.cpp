@@ -372,24 +372,54 @@ CaseConfig parse_case_toml(const std::string& text, const std::string& source) {
   {
     Reader r{doc.tables.at("velocity_grid"), "velocity_grid"};
     const TomlValue* aut = r.find("auto");
-    GSK_CHECK(aut == nullptr || (aut->kind == TomlValue::Bool && !aut->b),
-              "velocity_grid.auto is not supported; give n and range");
+    GSK_CHECK(aut == nullptr || aut->kind == TomlValue::Bool, "velocity_grid.auto must be true or false");
     const TomlValue* n = r.find("n");
     const TomlValue* range = r.find("range");
-    GSK_CHECK(n && n->kind == TomlValue::Array && n->arr.size() == 3, "velocity_grid.n must be [nx, ny, nz]");
-    GSK_CHECK(range && range->kind == TomlValue::Array && range->arr.size() == 3,
-              "velocity_grid.range must be [[lo, hi] x 3]");
     long long total = 1;
-    for (int a = 0; a < 3; ++a) {
-      GSK_CHECK(n->arr[static_cast<std::size_t>(a)].kind == TomlValue::Int, "velocity_grid.n must hold integers");
-      cfg.vn[static_cast<std::size_t>(a)] = static_cast<int>(n->arr[static_cast<std::size_t>(a)].i);
-      const TomlValue& ra = range->arr[static_cast<std::size_t>(a)];
-      GSK_CHECK(ra.kind == TomlValue::Array && ra.arr.size() == 2, "velocity_grid.range entries must be [lo, hi]");
-      cfg.vlo[static_cast<std::size_t>(a)] = ra.arr[0].number();
-      cfg.vhi[static_cast<std::size_t>(a)] = ra.arr[1].number();
-      GSK_CHECK(cfg.vlo[static_cast<std::size_t>(a)] < cfg.vhi[static_cast<std::size_t>(a)], "velocity_grid.range needs lo < hi");
-      GSK_CHECK(cfg.vn[static_cast<std::size_t>(a)] >= 1, "velocity_grid.n must be >= 1");
-      total *= cfg.vn[static_cast<std::size_t>(a)];
+    if (aut != nullptr && aut->b) {
+      // velocity_grid.auto (case_config.cpp:589-610): a cube [-L, L]^3 with L = u_max + 5 sqrt(R T_max)
+      // (the fastest wall plus five thermal speeds of the hottest boundary) and an even point count
+      // per axis, 2 ceil(L / (0.7 sqrt(R T_min))) (at least 2), so the coldest boundary Maxwellian
+      // has about 1.4 points per thermal speed; T_min / T_max over the boundary temperatures (T0
+      // when none is given), u_max over the wall velocities. In units of sqrt(R T0).
+      GSK_CHECK(n == nullptr && range == nullptr, "velocity_grid.auto = true cannot be combined with n or range");
+      double t_min = std::numeric_limits<double>::infinity(), t_max = 0.0, u_max = 0.0;
+      for (const BoundarySpec& b : cfg.boundaries) {
+        if (b.temperature) {
+          t_min = std::min(t_min, *b.temperature);
+          t_max = std::max(t_max, *b.temperature);
+        }
+        if (b.velocity) u_max = std::max(u_max, norm(*b.velocity));
+      }
+      if (t_max == 0.0) t_min = t_max = cfg.T0;
+      const double R = cfg.gas_constant();
+      const double limit = u_max + 5.0 * std::sqrt(R * t_max);
+      const double points = std::max(2.0, 2.0 * std::ceil(limit / (0.7 * std::sqrt(R * t_min))));
+      GSK_CHECK(std::isfinite(points) && points * points * points <= 100000000.0,
+                "velocity_grid.auto gives too many velocity points (at most 1e8)");
+      const double half = limit / std::sqrt(R * cfg.T0);
+      GSK_CHECK(std::isfinite(half), "velocity_grid.auto gives a range that is not finite");
+      for (std::size_t a = 0; a < 3; ++a) {
+        cfg.vn[a] = static_cast<int>(points);
+        cfg.vlo[a] = -half;
+        cfg.vhi[a] = half;
+        total *= cfg.vn[a];
+      }
+    } else {
+      GSK_CHECK(n && n->kind == TomlValue::Array && n->arr.size() == 3, "velocity_grid.n must be [nx, ny, nz]");
+      GSK_CHECK(range && range->kind == TomlValue::Array && range->arr.size() == 3,
+                "velocity_grid.range must be [[lo, hi] x 3]");
+      for (int a = 0; a < 3; ++a) {
+        GSK_CHECK(n->arr[static_cast<std::size_t>(a)].kind == TomlValue::Int, "velocity_grid.n must hold integers");
+        cfg.vn[static_cast<std::size_t>(a)] = static_cast<int>(n->arr[static_cast<std::size_t>(a)].i);
+        const TomlValue& ra = range->arr[static_cast<std::size_t>(a)];
+        GSK_CHECK(ra.kind == TomlValue::Array && ra.arr.size() == 2, "velocity_grid.range entries must be [lo, hi]");
+        cfg.vlo[static_cast<std::size_t>(a)] = ra.arr[0].number();
+        cfg.vhi[static_cast<std::size_t>(a)] = ra.arr[1].number();
+        GSK_CHECK(cfg.vlo[static_cast<std::size_t>(a)] < cfg.vhi[static_cast<std::size_t>(a)], "velocity_grid.range needs lo < hi");
+        GSK_CHECK(cfg.vn[static_cast<std::size_t>(a)] >= 1, "velocity_grid.n must be >= 1");
+        total *= cfg.vn[static_cast<std::size_t>(a)];
+      }
     }
     GSK_CHECK(total <= 100000000, "too many velocity points");
     // extension: tanh-stretched grid (no reference counterpart; 0 keeps the uniform midpoint grid)

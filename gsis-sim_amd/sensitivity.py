@@ -62,6 +62,7 @@ class SteadyProblem:
         self.boundary_iterations = int(params["boundary_iterations"])
         self.newton_tolerances = (solver.setup.view().deq_tolerance, 1e-10)
         self.log = None  # optional progress callback (message) for long solves
+        self.forward_solver = None  # set by use_forward_solver: the solver the forward solve runs on
         self.f0 = None
         self.R0 = None
         self.mdot0 = None
@@ -78,6 +79,42 @@ class SteadyProblem:
 
     def mdot(self) -> float:
         return -self.sv.patch_mass_flow()[self.inlet] * self.factor
+
+    def use_forward_solver(self, solver) -> None:
+        """Run the forward solve on another solver of the same case, typically the FP32 / mixed form
+        (FP32 f): forward() converges it and hands its f to this problem's FP64 solver, which then
+        carries the sensitivity solve at that state. The FP32 state solves the FP64 equations to the
+        forward residual target, so the sensitivities differ from those at the FP64 solution by
+        about that much relative to their size."""
+        self.forward_solver = solver
+
+    def forward(self, residual_target: float, max_iterations: int = 5000) -> float:
+        """The converged Mdot: on this problem's solver, or on the forward solver (then copied over)."""
+        if self.forward_solver is None:
+            return self.solve(residual_target=residual_target, max_iterations=max_iterations)
+        other = self.forward_solver
+        if self.delta_rp is not None:
+            other.set_mu0(1.0 / (math.sqrt(2.0) * self.delta_rp))
+        if self.alpha is not None and self.wall_patch is not None:
+            other.set_accommodation(self.wall_patch, self.alpha)
+        t0 = time.perf_counter()
+        it = 0
+        for it in range(max_iterations):
+            other.prepare()
+            info = other.sweep()
+            if self.log is not None and it % 25 == 0:
+                self.log(f"forward (element size {other.element_size()}) iteration {it}: residual {info['residual']:.3e}")
+            if info["residual"] <= residual_target:
+                break
+        other.prepare()
+        mdot = -other.patch_mass_flow()[self.inlet] * self.factor
+        self.sv.set_distribution(other.distribution())  # original numbering, grid order, as float64
+        self.sv.prepare()
+        self.stats["forward_seconds"] = time.perf_counter() - t0
+        self.stats["forward_iterations"] = it + 1
+        self.stats["forward_mdot"] = mdot
+        self.f0 = None
+        return mdot
 
     def solve(self, residual_target: float = 1e-10, max_iterations: int = 5000, init=None, log=None) -> float:
         """CIS iterations until the residual norm is below residual_target; returns Mdot."""
@@ -317,7 +354,7 @@ class ConvergedMdot(torch.autograd.Function):
     @staticmethod
     def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):
         problem.set_parameters(float(delta_rp), float(alpha) if problem.wall_patch is not None else None)
-        mdot = problem.solve(residual_target=residual_target)
+        mdot = problem.forward(residual_target)
         ctx.problem = problem
         return torch.tensor(mdot, dtype=torch.float64)
 
