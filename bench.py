@@ -197,33 +197,36 @@ def cpu_baseline(g, setup, iterations=1):
                        f"after 1 untimed ({dt:.1f} s timed, {t_first:.1f} s untimed)")
 
 
-def time_to_mdot(g, mg, args):
+TTM_FORMS = {"fp64": {"precision": "fp64"}, "fp32": {"precision": "fp32"},
+             "mixed": {"precision": "fp32", "moment_sums": "mfma"}}
+
+
+def time_to_mdot_runs(g, mg, forms, mesh="40k", delta=10.0, uniform_grid=False, residual_target=1e-6,
+                      max_iterations=800):
     """Metric 2: wall time until the inlet Mdot is within 1e-4 relative of the converged FP64 Mdot,
-    for the FP64, FP32 and mixed forms, on the 40k-cell tube with 24^3 velocities or (--ttm-mesh 135k)
-    the 135k-cell tube with 32^3, tanh-stretched unless --uniform-grid, delta_rp = --ttm-delta."""
+    for the given forms (fp64 first), on the 40k-cell tube with 24^3 velocities or the 135k-cell tube
+    with 32^3, tanh-stretched unless uniform_grid."""
     import gsis_sim_amd.cis_run as cr
 
-    big = args.ttm_mesh == "135k"
+    big = mesh == "135k"
     tube_args = WORKLOADS["tube135k_32cubed_fp32" if big else "tube40k_24cubed_fp64"][0]
-    n_vel, delta = (32 if big else 24), args.ttm_delta
+    n_vel = 32 if big else 24
     runs = {}
     with tempfile.TemporaryDirectory() as d:
-        for prec, kin in (("fp64", {"precision": "fp64"}), ("fp32", {"precision": "fp32"}),
-                          ("mixed", {"precision": "fp32", "moment_sums": "mfma"})):
-            path = mg.tube_case(d, n_vel=n_vel, vrange=6.0, delta=delta, kinetic=kin, name=f"tube_{prec}",
-                                stretch=0.0 if args.uniform_grid else 1.5, **tube_args)
+        for prec in forms:
+            path = mg.tube_case(d, n_vel=n_vel, vrange=6.0, delta=delta, kinetic=dict(TTM_FORMS[prec]), name=f"tube_{prec}",
+                                stretch=0.0 if uniform_grid else 1.5, **tube_args)
             setup = g.Setup(path)
             sv = g.Solver(setup)
             log(f"time-to-Mdot: {prec}")
-            h = cr.run(sv, setup.mass_flow_factor(), inlet=0, outlet=1, residual_target=args.residual_target,
-                       max_iterations=args.max_iterations, log=lambda m, p=prec: log(f"{p}: {m}"))
+            h = cr.run(sv, setup.mass_flow_factor(), inlet=0, outlet=1, residual_target=residual_target,
+                       max_iterations=max_iterations, log=lambda m, p=prec: log(f"{p}: {m}"))
             sv.close()
             runs[prec] = h
     ref = float(runs["fp64"]["mdot_in"][-1])
     out = {"metric": "time-to-Mdot (1e-4 rel)", "unit": "s",
-           "workload": f"tube{args.ttm_mesh}_{n_vel}cubed{'' if args.uniform_grid else '_tanh'}"
-                       + ("" if delta == 10.0 else f"_delta{delta:g}"), "mdot_ref_fp64": ref,
-           "residual_target": args.residual_target}
+           "workload": f"tube{mesh}_{n_vel}cubed{'' if uniform_grid else '_tanh'}" + ("" if delta == 10.0 else f"_delta{delta:g}"),
+           "mdot_ref_fp64": ref, "residual_target": residual_target}
     for prec, h in runs.items():
         t = cr.time_to_within(h, ref, 1e-4)
         out[prec] = {"iterations": int(h["iteration"][-1]) + 1, "stop": h["reason"], "final_residual": float(h["residual"][-1]),
@@ -231,10 +234,16 @@ def time_to_mdot(g, mg, args):
                      "rel_err_vs_fp64": abs(float(h["mdot_in"][-1]) - ref) / abs(ref),
                      "wall_s": float(h["t_wall"][-1]), "s_per_iteration": float(h["t_wall"][-1]) / len(h["t_wall"]),
                      "time_to_mdot_s": None if t is None else t[0], "iterations_to_mdot": None if t is None else t[1]}
-    if out["fp32"]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
-        out["speedup_fp32_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out["fp32"]["time_to_mdot_s"]
-    if out["mixed"]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
-        out["speedup_mixed_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out["mixed"]["time_to_mdot_s"]
+    for prec in forms:
+        if prec != "fp64" and out[prec]["time_to_mdot_s"] and out["fp64"]["time_to_mdot_s"]:
+            out[f"speedup_{prec}_vs_fp64"] = out["fp64"]["time_to_mdot_s"] / out[prec]["time_to_mdot_s"]
+    return out
+
+
+def time_to_mdot(g, mg, args):
+    out = time_to_mdot_runs(g, mg, ["fp64", "fp32", "mixed"], mesh=args.ttm_mesh, delta=args.ttm_delta,
+                            uniform_grid=args.uniform_grid, residual_target=args.residual_target,
+                            max_iterations=args.max_iterations)
     print(json.dumps(out), flush=True)
 
 
@@ -320,6 +329,8 @@ def main():
     ap.add_argument("--workload", default="tube40k_24cubed_tanh_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--uniform-grid", action="store_true", help="time-to-Mdot on the reference's uniform velocity grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-mdot", action="store_true",
+                    help="leave out the FP64-vs-mixed time-to-Mdot runs of the default workload's line")
     ap.add_argument("--cpu-iterations", type=int, default=1, help="timed CIS iterations of the CPU baseline")
     ap.add_argument("--time-to-mdot", action="store_true",
                     help="run FP64 and FP32 forms to convergence and report the time until the inlet Mdot is "
@@ -473,6 +484,9 @@ def main():
             "phase_seconds_per_step": {k: round(v / args.steps, 5) for k, v in phase.items()},
             "memory": memory,
         }
+        if not args.no_time_to_mdot and args.workload == "tube40k_24cubed_tanh_fp64":
+            # metric 2 on the same tube: the mixed form against FP64, both converged to residual 1e-6
+            result["time_to_mdot"] = time_to_mdot_runs(g, mg, ["fp64", "mixed"])
         if not args.no_cpu_baseline:
             log("cpu baseline")
             result["cpu_baseline"] = cpu_baseline(g, setup, iterations=args.cpu_iterations)

@@ -447,11 +447,17 @@ def write_cells_csv(path: str, setup: Setup, solver: Solver, desc: dict) -> None
 def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = None, min_iter: int = 1,
             tol: float | None = None, precision: str | None = None, overrides: dict | None = None,
             print_every: int = 0, cells_csv: bool = False, backend_factory=None, log=print,
-            init_from: str | None = None) -> dict:
+            init_from: str | None = None, checkpoint_every: int = 0, restart_from: str | None = None) -> dict:
     """CIS iterations to eq. (59) convergence (or max_iter); writes history.csv and summary.json
     into out_dir (default runs/kinetic/<case name>) and returns the summary. backend_factory(setup)
     gives an external backend (an ops table, or an object holding one as .ops, kept alive for the
-    run); the HIP backend when None."""
+    run); the HIP backend when None.
+
+    A file named STOP in out_dir ends the run after the current iteration, writing the results of
+    the current f and cells.csv (src/app/cmd_cis_run.cpp:324-343). checkpoint_every > 0 writes f
+    (original cell numbering, velocity-grid order, float64) to out_dir/checkpoint.npy every that many
+    iterations and when the run stops; restart_from starts from such a file (an extension: the
+    reference restarts from cells.csv, init_from here)."""
     if min_iter < 1:
         raise ValueError("min_iter must be at least 1")
     if max_iter is not None and max_iter < 1:
@@ -471,7 +477,12 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     tol = float(tol if tol is not None else desc["tolerance"])
     out_dir = out_dir or os.path.join("runs", "kinetic", desc["case"])
     os.makedirs(out_dir, exist_ok=True)
-    if init_from:
+    if restart_from:
+        f0 = np.load(restart_from, allow_pickle=False)
+        if f0.shape != (setup.n_cells, setup.nv):
+            raise ValueError(f"{restart_from} holds {f0.shape}, the case has {(setup.n_cells, setup.nv)}")
+        solver.set_distribution(f0)
+    elif init_from:
         solver.initialize_equilibrium(read_cells_csv(init_from, setup.n_cells))
     else:
         solver.initialize_uniform(params["initial_pressure"] / params["initial_temperature"], (0.0, 0.0, 0.0),
@@ -486,7 +497,12 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
     def section_flows():
         return list(solver.section_mass_flow(sec_pos, sec_axis)) if sec_pos else []
 
-    converged = False
+    def checkpoint():
+        tmp = os.path.join(out_dir, "checkpoint.tmp.npy")
+        np.save(tmp, solver.distribution().reshape(setup.n_cells, setup.nv))
+        os.replace(tmp, os.path.join(out_dir, "checkpoint.npy"))
+
+    converged = stopped = False
     timing_sum = dict.fromkeys(TIMING_KEYS, 0.0)
     timed, iter_time_sum = 0, 0.0
     last = None
@@ -517,14 +533,24 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
             if print_every > 0 and (info["iteration"] % print_every == 0 or info["iteration"] < 3):
                 log(f"iteration {info['iteration']:6d}  eps {info['epsilon'] if info['has_epsilon'] else 0.0:.3e}  "
                     f"residual {info['residual']:.3e}  {info['seconds']:.3f} s")
+            if checkpoint_every > 0 and (info["iteration"] + 1) % checkpoint_every == 0:
+                checkpoint()
+            if os.path.exists(os.path.join(out_dir, "STOP")):  # a gentle stop with the current f
+                stopped = True
+                last = solver.prepare()
+                history.write(last, solver.patch_mass_flow(), time.perf_counter() - t_total, section_flows())
+                log(f"STOP found: stopping at iteration {last['iteration']}")
+                break
     finally:
         history.close()
     wall_s = time.perf_counter() - t_total
 
     ps = summarize_patches(desc, setup.bface_geometry(), solver.boundary_fluxes())
     sec_final = section_flows()
-    if cells_csv:
+    if cells_csv or stopped:
         write_cells_csv(os.path.join(out_dir, "cells.csv"), setup, solver, desc)
+    if checkpoint_every > 0:
+        checkpoint()
     scale = 1.0 / timed if timed > 0 else 0.0
     kt = {k: timing_sum[k] * scale for k in TIMING_KEYS}
     kt["total"] = sum(timing_sum[k] for k in TIMING_KEYS) * scale
@@ -534,7 +560,7 @@ def cis_run(case_file: str, out_dir: str | None = None, max_iter: int | None = N
         "case": desc["case"], "case_file": str(case_file), "mesh": desc["mesh"],
         "n_cells": desc["n_cells"], "n_boundary_faces": desc["n_boundary_faces"], "n_velocity": desc["n_velocity"],
         "Kn": desc["Kn"], "delta": desc["delta"],
-        "iterations": int(last["iteration"]), "converged": converged, "stopped": False, "stopped_by_user": False,
+        "iterations": int(last["iteration"]), "converged": converged, "stopped": stopped, "stopped_by_user": stopped,
         "tolerance": tol,
         "epsilon": last["epsilon"] if eps else None, "eps_rho": last["eps_rho"] if eps else None,
         "eps_u": last["eps_u"] if eps else None, "eps_T": last["eps_T"] if eps else None,
@@ -633,6 +659,8 @@ def main(argv=None) -> int:
     ap.add_argument("--print-every", type=int, default=100)
     ap.add_argument("--cells-csv", action="store_true")
     ap.add_argument("--init-from", help="restart from the cell values of a previous cells.csv")
+    ap.add_argument("--checkpoint-every", type=int, default=0, help="write f to <out>/checkpoint.npy every N iterations")
+    ap.add_argument("--restart-from", help="start from f in a checkpoint.npy of a previous run")
     a = ap.parse_args(argv)
     ov = {}
     for kv in a.set:
@@ -641,7 +669,7 @@ def main(argv=None) -> int:
             ap.error(f"--set expects key=value, got {kv!r}")
         ov[k.strip()] = _parse_value(v.strip())
     s = cis_run(a.case, a.out, a.max_iter, a.min_iter, a.tol, a.precision, ov, a.print_every, a.cells_csv,
-                init_from=a.init_from)
+                init_from=a.init_from, checkpoint_every=a.checkpoint_every, restart_from=a.restart_from)
     print(f"iterations {s['iterations']} ({'converged' if s['converged'] else 'limit reached'}), "
           f"{s['wall_time_s']:.2f} s")
     return 0

@@ -220,3 +220,41 @@ def test_restart_from_cells_csv(gsk, meshgen, oracle_mod, cr, tmp_path):
     open(str(tmp_path / "nocol.csv"), "w").write(rows[0].replace(",T,", ",Temp,") + "\n")
     with pytest.raises(ValueError, match="no column"):
         cr.read_cells_csv(str(tmp_path / "nocol.csv"), 30)
+
+
+def test_stop_file_ends_the_run_with_results(gsk, meshgen, oracle_mod, cr, tmp_path):
+    """A STOP file in the output directory ends the run after the current iteration: the summary
+    says stopped (by the user), and cells.csv is written although it was not asked for
+    (src/app/cmd_cis_run.cpp:324-343)."""
+    case = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"})
+    out = tmp_path / "run"
+    out.mkdir()
+    (out / "STOP").write_text("")
+    s = _run(gsk, cr, oracle_mod, case, str(out), max_iter=50)
+    assert s["stopped"] and s["stopped_by_user"] and not s["converged"]
+    assert s["iterations"] == 1
+    assert (out / "cells.csv").exists()
+    _, rows = _history(str(out / "history.csv"))
+    assert len(rows) == 2  # the sweep of iteration 0 and the final state
+
+
+def test_checkpoint_and_restart(gsk, meshgen, oracle_mod, cr, tmp_path):
+    """checkpoint_every writes f (original numbering, grid order) that restart_from reads back: the
+    restarted solver holds exactly the checkpointed f, and a restarted run continues from it."""
+    case = box_case(meshgen, str(tmp_path), kinetic={"precision": "fp64"})
+    fac = lambda st: oracle_mod.CpuBackend(st, "oracle", ops_type=gsk.BackendOps)  # noqa: E731
+    _run(gsk, cr, oracle_mod, case, str(tmp_path / "a"), max_iter=4, checkpoint_every=2)
+    ck = str(tmp_path / "a" / "checkpoint.npy")
+    f = np.load(ck)
+    setup = cr.Setup(case)
+    assert f.shape == (setup.n_cells, setup.nv)
+    keep = fac(setup)
+    sv = cr.Solver(setup, keep.ops)
+    sv.set_distribution(f)
+    assert np.array_equal(sv.distribution().reshape(f.shape), f)
+    sv.close()
+    s = cr.cis_run(case, str(tmp_path / "b"), max_iter=2, backend_factory=fac, restart_from=ck)
+    assert s["iterations"] == 2
+    with pytest.raises(ValueError, match="holds"):
+        np.save(str(tmp_path / "bad.npy"), f[:-1])
+        cr.cis_run(case, str(tmp_path / "c"), max_iter=1, backend_factory=fac, restart_from=str(tmp_path / "bad.npy"))
