@@ -12,8 +12,17 @@ A step is one CIS iteration (prepare + sweep). One update = one (cell, velocity)
 inner LU-SGS sweep (gradient + residual + block passes + f update), so a step does
 cells x velocities x inner_sweeps updates. Inputs are resident in HBM before the timed region.
 
-Prints ONE JSON line; per-kernel HIP-event timings of the backend's own stream give the roofline of
-the dominant kernel; the CPU oracle is timed on the host on a bounded sample of the same workload.
+Prints ONE JSON line: the step rate, the HBM roofline of one CIS iteration (its algorithmic bytes
+per update, (2 + 2 block_passes) sizeof(f), plus one f read for the moment pass, over the measured
+step time; the HBM traffic the committed rocprofv3 summary of the workload counts beside it), the
+dominant kernel from HIP events on the backend's own stream, and for the default workload the time
+to Mdot of the FP64 and mixed forms on the same tube (metric 2). The CPU oracle (OpenMP, all the
+host threads the process may use) is timed on the same workload for one CIS iteration after an
+untimed one.
+
+Other modes: --time-to-mdot (FP64 / FP32 / mixed, 40k or 135k tube), --sensitivity (metric 3:
+sensitivity cost relative to the forward solve, with --fd-check central differences), and the
+--summarize-* helpers that turn rocprofv3 CSVs into the summaries kept under profiles/.
 """
 
 from __future__ import annotations

@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double*
 }
 
 // specular caches of partially accommodating walls (k_maxwell_reflection)
-template <class F, bool Reconstruct>
+template <class F, bool Reconstruct, int Mode = 0>  // Mode: limiter branches (limited_gradient)
 __global__ void k_maxwell_reflection(View<F> v, F* __restrict__ out) {
   const int b = blockIdx.y;
   if (b >= v.n_bfaces || v.mw_off[b] < 0) return;
@@ -443,7 +443,8 @@ __global__ void k_maxwell_reflection(View<F> v, F* __restrict__ out) {
   if (p >= v.nv) return;
   const BFace<F>& bf = v.bf[b];
   const F x = v.xi_x[p], y = v.xi_y[p], z = v.xi_z[p];
-  out[v.mw_off[b] + p] = (x * bf.nx + y * bf.ny + z * bf.nz) < F(0) ? reflected_value<Reconstruct>(v, bf, p, x, y, z) : F(0);
+  out[v.mw_off[b] + p] =
+      (x * bf.nx + y * bf.ny + z * bf.nz) < F(0) ? reflected_value<Reconstruct, Mode>(v, bf, p, x, y, z) : F(0);
 }
 
 // 20 half-space sums per boundary face in FP64 (k_boundary_sums): flux and moment sums (5 each)
@@ -452,7 +453,7 @@ __global__ void k_maxwell_reflection(View<F> v, F* __restrict__ out) {
 // block holds 10 accumulators); the velocities of a face are split over gridDim.y blocks (block y
 // takes every gridDim.y-th run of blockDim velocities). k_boundary_reduce adds the partials in order.
 constexpr int kSideSums = 10;
-template <class F>
+template <class F, int Mode = 0>  // Mode: limiter branches (limited_gradient)
 __global__ __launch_bounds__(kBlock) void k_boundary_sums(View<F> v, const int* __restrict__ faces, int n_faces,
                                                           double* __restrict__ part) {
   __shared__ double lds[kWaves * kSideSums];
@@ -471,10 +472,10 @@ __global__ __launch_bounds__(kBlock) void k_boundary_sums(View<F> v, const int* 
       if (outgoing != out_side) continue;
       F fb;
       if (outgoing) {
-        fb = reconstruct_at_boundary(v, bf, p, x, y, z);
+        fb = reconstruct_at_boundary<Mode>(v, bf, p, x, y, z);
       } else if (bf.kind == kBcSymmetry) {
         const int pm = p ^ bf.mirror_xor;
-        fb = reconstruct_at_boundary(v, bf, pm, v.xi_x[pm], v.xi_y[pm], v.xi_z[pm]);
+        fb = reconstruct_at_boundary<Mode>(v, bf, pm, v.xi_x[pm], v.xi_y[pm], v.xi_z[pm]);
       } else if (bf.kind == kBcMaxwell) {
         fb = maxwell_inflow(v, b, p, x, y, z);
       } else {
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int
 }
 
 // cells with more than kGather faces, from the CSR (after k_gradient, which wrote them wrongly)
-template <class F>
+template <class F, int Mode = 0>  // Mode: limiter branches (limited_gradient)
 __global__ __launch_bounds__(kBlock) void k_gradient_csr(View<F> v, Chunk<F> ch, const int* __restrict__ cells, int n,
                                                          int tiles) {
   for (i64 L = blockIdx.x; L < static_cast<i64>(n) * tiles; L += gridDim.x) {
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient_csr(View<F> v, Chunk<F> ch,
     if (t >= ch.kc) continue;
     const int p = ch.k0 + t;
     F gx, gy, gz;
-    limited_gradient(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
+    limited_gradient<Mode>(v, c, p, v.xi_x[p], v.xi_y[p], v.xi_z[p], f_at(v, c, p), gx, gy, gz);
     const i64 wi = static_cast<i64>(c) * ch.stride + t;
     ch.gx[wi] = gx;
     ch.gy[wi] = gy;
@@ -1684,7 +1685,7 @@ class HipBackend final : public Backend {
     if (n_mw_ > 0) {
       const dim3 grid((nv_ + 255) / 256, nb_);
       dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
-      dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+      launch_reflection(grid);
     }
     if (nb_ > 0) launch_boundary_sums(nullptr, nb_);
     HIPCHK(hipGetLastError());
@@ -1712,7 +1713,7 @@ class HipBackend final : public Backend {
     if (n_mw_ > 0) {
       const dim3 grid((nv_ + 255) / 256, nb_);
       dev::k_maxwell_reflection<F, false><<<grid, 256, 0, stream_>>>(view(), specular_first_.p);
-      dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+      launch_reflection(grid);
     }
     launch_boundary_sums(faces_dev, static_cast<int>(n_faces));
   }
@@ -1720,7 +1721,12 @@ class HipBackend final : public Backend {
   // partial sums over velocity chunks, then their fixed-order sum into bsums_
   void launch_boundary_sums(const int* faces, int n_faces) {
     const dim3 grid(static_cast<unsigned>(std::min(n_faces, 65535)), static_cast<unsigned>(bsum_chunks_), 2);
-    dev::k_boundary_sums<F><<<grid, dev::kBlock, 0, stream_>>>(view(), faces, n_faces, bsum_part_.p);
+    if (lim_mode_ == 1)
+      dev::k_boundary_sums<F, 1><<<grid, dev::kBlock, 0, stream_>>>(view(), faces, n_faces, bsum_part_.p);
+    else if (lim_mode_ == 2)
+      dev::k_boundary_sums<F, 2><<<grid, dev::kBlock, 0, stream_>>>(view(), faces, n_faces, bsum_part_.p);
+    else
+      dev::k_boundary_sums<F><<<grid, dev::kBlock, 0, stream_>>>(view(), faces, n_faces, bsum_part_.p);
     const int nj = n_faces * dev::kBoundarySums;
     dev::k_boundary_reduce<<<(nj + 255) / 256, 256, 0, stream_>>>(faces, n_faces, bsum_chunks_, bsum_part_.p, bsums_.p);
   }
@@ -1736,6 +1742,16 @@ class HipBackend final : public Backend {
   // Workgroups of one launch over `work` items: at most grid_cap_ (a multiple of 8, so a kernel's
   // L mod 8 stays its XCD), keeping blocks x threads far below the 2^32 a launch allows; the
   // gradient / residual cover larger work with several launches, the CSR-path kernels loop.
+  // the reconstructed specular cache of the Maxwell walls, on the current limiter-branch mode
+  void launch_reflection(dim3 grid) {
+    if (lim_mode_ == 1)
+      dev::k_maxwell_reflection<F, true, 1><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+    else if (lim_mode_ == 2)
+      dev::k_maxwell_reflection<F, true, 2><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+    else
+      dev::k_maxwell_reflection<F, true><<<grid, 256, 0, stream_>>>(view(), specular_.p);
+  }
+
   unsigned capped_grid(long long work) const {
     const long long g = std::min<long long>(std::max<long long>(work, 8), grid_cap_);
     return static_cast<unsigned>((g + 7) / 8 * 8);
@@ -1756,7 +1772,14 @@ class HipBackend final : public Backend {
         dev::k_gradient<F><<<grid, dev::kBlock, 0, stream_>>>(v, ch, tiles_, l0);
     }
     if (n_irregular_ > 0)
-      dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
+    {
+      if (v.lim_mode == 1)
+        dev::k_gradient_csr<F, 1><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
+      else if (v.lim_mode == 2)
+        dev::k_gradient_csr<F, 2><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
+      else
+        dev::k_gradient_csr<F><<<grid_irr, dev::kBlock, 0, stream_>>>(v, ch, irregular_.p, n_irregular_, tiles_);
+    }
     HIPCHK(hipGetLastError());
     end();
     begin(kResidual);

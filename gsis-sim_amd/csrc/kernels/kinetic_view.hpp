@@ -202,12 +202,25 @@ __device__ __forceinline__ void limit_and_scale(const View<F>& v, int cell, cons
 constexpr unsigned kNoCandidate = 15;
 constexpr int kLimMaxFaces = 15;
 
-// phi * grad for one (cell, velocity) from its n face values fk (faces whose bit is set in `use`
-// count for the extremes) and face offsets ds, recording (Mode 1) or following (Mode 2) the branch.
-template <int Mode, class F>
-__device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, const F* fk, unsigned use,
-                                const F* dsx, const F* dsy, const F* dsz, F gx, F gy, F gz, F& ogx, F& ogy,
-                                F& ogz) {
+// Element k of a register array, with k a run-time value: an unrolled select, so the array stays in
+// registers (an indexed access would place it in scratch memory).
+template <int N, class F>
+__device__ __forceinline__ F pick(const F (&a)[N], unsigned k) {
+  F r = a[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i)
+    if (k == static_cast<unsigned>(i)) r = a[i];
+  return r;
+}
+
+// phi * grad for one (cell, velocity) from its N face slots (fk the face values; faces whose bit is
+// set in `use` count for the extremes) and face offsets ds, recording (Mode 1) or following (Mode 2)
+// the branch.
+template <int Mode, int N, class F>
+__device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, const F (&fk)[N], unsigned use,
+                                const F (&dsx)[N], const F (&dsy)[N], const F (&dsz)[N], F gx, F gy, F gz, F& ogx,
+                                F& ogy, F& ogz) {
+  static_assert(N <= kLimMaxFaces, "a branch record holds at most kLimMaxFaces faces");
   unsigned* slot = v.lim_branch + static_cast<i64>(cell) * v.nv + p;
   if (v.limiter_mode == kLimNone && v.positivity == 0) {
     ogx = gx;
@@ -224,7 +237,8 @@ __device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, 
   unsigned kmx = kNoCandidate, kmn = kNoCandidate, imx = kNoCandidate, imn = kNoCandidate;
   F fmx = fi, fmn = fi, proj_max = F(0), proj_min = F(0);
   if (Mode == 1) {
-    for (int k = 0; k < n; ++k) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
       if (!((use >> k) & 1u)) continue;
       if (fk[k] > fmx) {
         fmx = fk[k];
@@ -235,7 +249,8 @@ __device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, 
         imn = static_cast<unsigned>(k);
       }
     }
-    for (int k = 0; k < n; ++k) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
       if (!((use >> k) & 1u)) continue;
       const F proj = dot3(gx, gy, gz, dsx[k], dsy[k], dsz[k]);
       if (proj > proj_max) {
@@ -252,10 +267,10 @@ __device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, 
     kmn = (code >> 4) & 15u;
     imx = (code >> 8) & 15u;
     imn = (code >> 12) & 15u;
-    if (imx != kNoCandidate) fmx = fk[imx];
-    if (imn != kNoCandidate) fmn = fk[imn];
-    if (kmx != kNoCandidate) proj_max = dot3(gx, gy, gz, dsx[kmx], dsy[kmx], dsz[kmx]);
-    if (kmn != kNoCandidate) proj_min = dot3(gx, gy, gz, dsx[kmn], dsy[kmn], dsz[kmn]);
+    if (imx != kNoCandidate) fmx = pick(fk, imx);
+    if (imn != kNoCandidate) fmn = pick(fk, imn);
+    if (kmx != kNoCandidate) proj_max = dot3(gx, gy, gz, pick(dsx, kmx), pick(dsy, kmx), pick(dsz, kmx));
+    if (kmn != kNoCandidate) proj_min = dot3(gx, gy, gz, pick(dsx, kmn), pick(dsy, kmn), pick(dsz, kmn));
   }
   F eps2 = v.eps2[cell];
   bool eps_from_max = false;
@@ -303,16 +318,19 @@ __device__ void limit_on_branch(const View<F>& v, int cell, int p, F fi, int n, 
   ogz = phi * gz;
 }
 
-// The limited gradient from the CSR entries (any face count up to kLimMaxFaces), recording or
-// following the branch.
-template <int Mode, class F>
-__device__ void limited_gradient_on_branch(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy,
-                                           F& ogz) {
+// The limited gradient from the CSR entries (cells of at most kGather faces in registers, larger
+// ones up to kLimMaxFaces), recording or following the branch.
+template <int Mode, int N, class F>
+__device__ __forceinline__ void limited_gradient_on_branch_n(const View<F>& v, int cell, int p, F x, F y, F z, F fi,
+                                                             F& ogx, F& ogy, F& ogz) {
   const int e0 = v.csr_off[cell], n = v.csr_off[cell + 1] - e0;
-  F fk[kLimMaxFaces], dsx[kLimMaxFaces], dsy[kLimMaxFaces], dsz[kLimMaxFaces];
+  F fk[N], dsx[N], dsy[N], dsz[N];
   unsigned use = 0;
   F gx = F(0), gy = F(0), gz = F(0);
-  for (int k = 0; k < n && k < kLimMaxFaces; ++k) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    fk[k] = dsx[k] = dsy[k] = dsz[k] = F(0);
+    if (k >= n) continue;
     const CsrE<F>& e = v.csr[e0 + k];
     fk[k] = e.other >= 0 ? f_at(v, e.other, p) : boundary_value_first_order(v, e, cell, p, x, y, z, fi);
     const F d = fk[k] - fi;
@@ -324,8 +342,24 @@ __device__ void limited_gradient_on_branch(const View<F>& v, int cell, int p, F 
     dsz[k] = e.dsz;
     if (!(e.other < 0 && v.bf[e.bface].kind == kBcPeriodic)) use |= 1u << k;
   }
-  limit_on_branch<Mode>(v, cell, p, fi, n < kLimMaxFaces ? n : kLimMaxFaces, fk, use, dsx, dsy, dsz, gx, gy, gz, ogx,
-                        ogy, ogz);
+  limit_on_branch<Mode>(v, cell, p, fi, fk, use, dsx, dsy, dsz, gx, gy, gz, ogx, ogy, ogz);
+}
+
+// cells with more than kGather faces (none on hexahedral, prism or tetrahedral meshes): out of line,
+// so that their wider arrays do not weigh on the common path's registers
+template <int Mode, class F>
+__device__ __attribute__((noinline)) void limited_gradient_on_branch_wide(const View<F>& v, int cell, int p, F x, F y,
+                                                                           F z, F fi, F& ogx, F& ogy, F& ogz) {
+  limited_gradient_on_branch_n<Mode, kLimMaxFaces>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
+}
+
+template <int Mode, class F>
+__device__ void limited_gradient_on_branch(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy,
+                                           F& ogz) {
+  if (v.csr_off[cell + 1] - v.csr_off[cell] <= kGather)
+    limited_gradient_on_branch_n<Mode, kGather>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
+  else
+    limited_gradient_on_branch_wide<Mode>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
 }
 
 // gradient from at most kGather entries ents[0..n): neighbour values gathered first, then combined
@@ -357,14 +391,12 @@ __device__ __forceinline__ void gradient_gathered(const View<F>& v, int cell, in
   limit_and_scale(v, cell, ents, n, fi, fmx, fmn, gx, gy, gz, ogx, ogy, ogz);
 }
 
-template <class F>
+// Mode: limiter branches, a compile-time choice so that the normal kernels (Mode 0) carry none of
+// the branch code: 1 records the branch, 2 follows the record (limited_gradient_on_branch).
+template <int Mode = 0, class F>
 __device__ void limited_gradient(const View<F>& v, int cell, int p, F x, F y, F z, F fi, F& ogx, F& ogy, F& ogz) {
-  if (v.lim_mode == 1) {
-    limited_gradient_on_branch<1>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
-    return;
-  }
-  if (v.lim_mode == 2) {
-    limited_gradient_on_branch<2>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
+  if constexpr (Mode != 0) {
+    limited_gradient_on_branch<Mode>(v, cell, p, x, y, z, fi, ogx, ogy, ogz);
     return;
   }
   const int e0 = v.csr_off[cell], e1 = v.csr_off[cell + 1];
@@ -457,26 +489,26 @@ __device__ void limited_gradient_row_on_branch(const View<F>& v, const EllRow<F>
     gz = madd(gz, r.cz[k], d);
     use |= 1u << k;
   }
-  limit_on_branch<Mode>(v, cell, p, fi, kGather, fk, use, r.dsx, r.dsy, r.dsz, gx, gy, gz, ogx, ogy, ogz);
+  limit_on_branch<Mode>(v, cell, p, fi, fk, use, r.dsx, r.dsy, r.dsz, gx, gy, gz, ogx, ogy, ogz);
 }
 
 // outgoing reconstructed value at a boundary face (kinetic_kernels.hpp:204-211)
-template <class F>
+template <int Mode = 0, class F>
 __device__ F reconstruct_at_boundary(const View<F>& v, const BFace<F>& b, int p, F x, F y, F z) {
   const F fi = f_at(v, b.owner, p);
   if (b.kind == kBcPeriodic) return fi;
   F gx, gy, gz;
-  limited_gradient(v, b.owner, p, x, y, z, fi, gx, gy, gz);
+  limited_gradient<Mode>(v, b.owner, p, x, y, z, fi, gx, gy, gz);
   return fi + gx * b.dx + gy * b.dy + gz * b.dz;
 }
 
 // specular reflection about the face normal, trilinear in velocity with a Maxwellian-ratio weight
 // (kinetic_kernels.hpp:80-127)
-template <bool Reconstruct, class F>
+template <bool Reconstruct, int Mode = 0, class F>
 __device__ F reflected_value(const View<F>& v, const BFace<F>& b, int p, F x, F y, F z) {
   if (b.mirror_xor != 0) {
     const int q = p ^ b.mirror_xor;
-    if (Reconstruct) return reconstruct_at_boundary(v, b, q, v.xi_x[q], v.xi_y[q], v.xi_z[q]);
+    if (Reconstruct) return reconstruct_at_boundary<Mode>(v, b, q, v.xi_x[q], v.xi_y[q], v.xi_z[q]);
     return f_at(v, b.owner, q);
   }
   const F un = x * b.nx + y * b.ny + z * b.nz;
@@ -522,7 +554,7 @@ __device__ F reflected_value(const View<F>& v, const BFace<F>& b, int p, F x, F 
     const F qx = v.xi_x[q], qy = v.xi_y[q], qz = v.xi_z[q];
     F value;
     if (Reconstruct)
-      value = reconstruct_at_boundary(v, b, q, qx, qy, qz);
+      value = reconstruct_at_boundary<Mode>(v, b, q, qx, qy, qz);
     else
       value = f_at(v, b.owner, q);
     const double drx = static_cast<double>(r[0]) - ref.ux, dry = static_cast<double>(r[1]) - ref.uy,

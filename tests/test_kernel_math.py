@@ -262,7 +262,7 @@ void run(bool stretched) {
     for (int p = 0; p < nv; ++p) {
       const F x = xs[p], y = ys[p], z = zs[p];
       if (!(x * bf.nx + y * bf.ny + z * bf.nz < F(0))) continue;
-      const F got = reflected_value<false, F>(v, bf, p, x, y, z);
+      const F got = reflected_value<false, 0, F>(v, bf, p, x, y, z);
       const double un = double(x) * double(bf.nx) + double(y) * double(bf.ny) + double(z) * double(bf.nz);
       const double r[3] = {double(x) - 2.0 * un * double(bf.nx), double(y) - 2.0 * un * double(bf.ny),
                            double(z) - 2.0 * un * double(bf.nz)};
