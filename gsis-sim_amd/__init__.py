@@ -122,7 +122,8 @@ EXPORTS = [
     "gsk_solver_residual_per_cell", "gsk_solver_section_mass_flow", "gsk_solver_section_cuts", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
     "gsk_solver_set_distribution_device", "gsk_solver_residual_device", "gsk_solver_precondition_device",
     "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
-    "gsk_solver_set_boundary_iterations", "gsk_solver_freeze_limiters",
+    "gsk_solver_set_boundary_iterations", "gsk_solver_freeze_limiters", "gsk_solver_adjoint_prepare",
+    "gsk_solver_adjoint_apply_device",
 ]
 
 # fixed device scratch of gsk_backend_gsis_correction_noalloc (GSK_NOALLOC_SCRATCH_BYTES, GSK_NOALLOC_MAX_CELLS)
@@ -489,6 +490,16 @@ class Solver:
         """Record every limiter branch at the next residual_device and follow it afterwards (on), or
         restore the normal limiter (off)."""
         _check(lib().gsk_solver_freeze_limiters(self._h, C.c_int32(1 if on else 0)))
+
+    def adjoint_prepare(self) -> None:
+        """Tabulate the transposed Jacobian's tables at the current f (limiter branches recorded there)."""
+        _check(lib().gsk_solver_adjoint_prepare(self._h))
+
+    def adjoint_apply_device(self, lam_ptr: int, patch_weights, out_ptr: int, n: int) -> None:
+        """out = J^T lam + (d/df sum_k w_k patch_mass_flow_k)^T (lam_ptr 0: lam = 0)."""
+        w = (C.c_double * len(patch_weights))(*[float(x) for x in patch_weights])
+        _check(lib().gsk_solver_adjoint_apply_device(self._h, C.c_void_p(lam_ptr or None), w, C.c_size_t(len(w)),
+                                                     C.c_void_p(out_ptr), C.c_size_t(n)))
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and _lib_handle is not None:

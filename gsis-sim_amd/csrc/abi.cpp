@@ -720,6 +720,23 @@ int gsk_solver_freeze_limiters(gsk_solver* sv, int32_t on) {
   });
 }
 
+int gsk_solver_adjoint_prepare(gsk_solver* sv) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->adjoint_prepare();
+  });
+}
+
+int gsk_solver_adjoint_apply_device(gsk_solver* sv, const void* lam, const double* patch_weights, size_t n_patches,
+                                    void* out, size_t n) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(patch_weights, "patch_weights");
+    need(out, "out");
+    sv->sv->adjoint_apply_device(lam, patch_weights, n_patches, out, n);
+  });
+}
+
 int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n) {
   return guarded([&] {
     need(sv, "solver");

@@ -97,6 +97,19 @@ class Backend {
   // limiter branches of every limited gradient: 0 off, 1 record at the next residual evaluation and
   // follow the record afterwards, 2 follow the last record
   virtual void set_limiter_branches(int mode) { unsupported(); }
+  // Adjoint operators (FP64 form), at the current f with the limiter branches followed.
+  // adjoint_prepare tabulates the cell parameters' Jacobians. adjoint_stage1 writes to out the
+  // transposed residual applied to lam (device arrays; lam may be null for zero) and returns, per
+  // boundary face, the adjoint of its inflow parameters (scale, A, B, ux, uy, uz) from the residual
+  // (inflow_adj, 6 per face). adjoint_stage2 then adds to out the transposed boundary sums
+  // (sums_adj, 20 per face), moments (moments_adj, 15 per cell, may be null) and what stage 1 left
+  // in the backend (gradients, reflections, cell parameters).
+  virtual void adjoint_prepare() { unsupported(); }
+  virtual void adjoint_stage1(const void* lam, void* out, double* inflow_adj, std::size_t n) { unsupported(); }
+  virtual void adjoint_stage2(const double* sums_adj, std::size_t n_sums, const double* moments_adj, std::size_t n_m,
+                              void* out) {
+    unsupported();
+  }
 
  private:
   [[noreturn]] void unsupported() const {

@@ -80,6 +80,15 @@ class KineticSolver {
   // current f, and later evaluations follow that record, so R is smooth around the recorded state;
   // off = the normal limiter.
   void freeze_limiters(bool on);
+  // Adjoint operators (FP64 form; no counterpart in the reference). adjoint_prepare, at the current f
+  // whose limiter branches freeze_limiters has recorded: tabulates what the transposed Jacobian of
+  // R(f) needs there (Jacobians of the boundary model's face maps and of the cell parameters).
+  // adjoint_apply_device: out = J^T lam + (d/df sum_k w_k patch_mass_flow_k)^T at that state, w one
+  // weight per patch, lam and out device [cell][storage position] arrays (lam may be null: 0); valid
+  // until the state changes (a new f, a residual evaluation, a parameter).
+  void adjoint_prepare();
+  void adjoint_apply_device(const void* lam, const double* patch_weights, std::size_t n_patches, void* out,
+                            std::size_t n);
 
  private:
   void reset_after_initialization();
@@ -100,6 +109,9 @@ class KineticSolver {
   double inflow_tol_ = 1e-10;
   long long boundary_iterations_ = 1;
   enum class Freeze { Off, Record, Follow } freeze_ = Freeze::Off;
+  struct AdjointTables;  // adjoint.cpp
+  std::shared_ptr<AdjointTables> adj_;
+  long long adj_version_ = -1;
   std::vector<double> specular_;  // 1 - alpha per boundary face (Maxwell walls)
   gsk_iteration_info info_{};
   std::vector<double> moments_, prev_moments_;  // 15 per cell, internal numbering

@@ -400,6 +400,35 @@ __global__ void k_scale_moments(double* __restrict__ moments, int n_cells, doubl
 
 // Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
 // Shakhov-polynomial correction gamma when enabled (k_cell_params_deq)
+// Cell parameters of one cell from its 15 moments, all threads of the block cooperating on the
+// discrete-equilibrium sums; r: the Newton's iterations (0 resolved, -1 failed)
+template <class F>
+__device__ CellP<F> cell_params_block(const View<F>& v, const double* m, double volume, double vdt, double mu0,
+                                      double omega, double Pr, int& r, double* lds) {
+  CellP<F> cp = make_cell_params<F>(m, volume, vdt, mu0, omega, Pr, v.cold_bgk);
+  r = 0;
+  if (v.discrete_eq) {
+    Maxw<F> eq;
+    double gamma[5];
+    r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
+                                  m[2], m[3], m[4], &cp.s, eq, gamma, lds);
+    if (r > 0) {
+      cp.s.eq = eq;
+      cp.s.g0 = static_cast<F>(gamma[0]);
+      cp.s.gx = static_cast<F>(gamma[1]);
+      cp.s.gy = static_cast<F>(gamma[2]);
+      cp.s.gz = static_cast<F>(gamma[3]);
+      cp.s.g4 = static_cast<F>(gamma[4]);
+    } else if (r < 0 && v.deq_fallback) {
+      cp.coll = F(0);
+      cp.diag = static_cast<F>(vdt);
+    }
+  }
+  return cp;
+}
+
+// Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
+// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq)
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double* __restrict__ moments,
                                                         const double* __restrict__ volume, const double* __restrict__ vdt,
@@ -407,26 +436,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double*
                                                         int* __restrict__ flags) {
   __shared__ double lds[kWaves * kDeqSums];
   for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
-    const double* m = moments + static_cast<i64>(c) * kMoments;
-    CellP<F> cp = make_cell_params<F>(m, volume[c], vdt[c], mu0, omega, Pr, v.cold_bgk);
     int r = 0;
-    if (v.discrete_eq) {
-      Maxw<F> eq;
-      double gamma[5];
-      r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
-                                    m[2], m[3], m[4], &cp.s, eq, gamma, lds);
-      if (r > 0) {
-        cp.s.eq = eq;
-        cp.s.g0 = static_cast<F>(gamma[0]);
-        cp.s.gx = static_cast<F>(gamma[1]);
-        cp.s.gy = static_cast<F>(gamma[2]);
-        cp.s.gz = static_cast<F>(gamma[3]);
-        cp.s.g4 = static_cast<F>(gamma[4]);
-      } else if (r < 0 && v.deq_fallback) {
-        cp.coll = F(0);
-        cp.diag = static_cast<F>(vdt[c]);
-      }
-    }
+    const CellP<F> cp =
+        cell_params_block(v, moments + static_cast<i64>(c) * kMoments, volume[c], vdt[c], mu0, omega, Pr, r, lds);
     if (threadIdx.x == 0) {
       cells[c] = cp;
       flags[c] = r;
@@ -1207,6 +1219,11 @@ __global__ void k_to_double(const F* __restrict__ src, double* __restrict__ dst,
 }
 
 }  // namespace dev
+}  // namespace gsk
+
+#include "kinetic_adjoint.hpp"
+
+namespace gsk {
 
 // ==============================================================================================
 // Host-side backend
@@ -1335,6 +1352,33 @@ class HipBackend final : public Backend {
                 "CSR entry refers to a cell or face out of range");
     }
     csr_.upload(csr.data(), csr.size(), stream_);
+    // adjoint tables: the slot of every internal entry's reverse entry in the neighbour's row, each
+    // boundary face's slot in its owner's row, and the owners of boundary faces
+    adj_mate_host_.assign(static_cast<std::size_t>(n_entries), -1);
+    adj_bslot_host_.assign(static_cast<std::size_t>(nb_), -1);
+    for (int c = 0; c < nc_; ++c)
+      for (int k = s.csr_offsets[c]; k < s.csr_offsets[c + 1]; ++k) {
+        const gsk_csr_entry& e = s.csr[k];
+        if (e.other < 0) {
+          adj_bslot_host_[static_cast<std::size_t>(e.bface)] = k - s.csr_offsets[c];
+          continue;
+        }
+        int best = -1;
+        double best_d = 0.0;
+        for (int m = s.csr_offsets[e.other]; m < s.csr_offsets[e.other + 1]; ++m) {
+          if (s.csr[m].other != c) continue;
+          const double d = std::fabs(s.csr[m].s[0] + e.s[0]) + std::fabs(s.csr[m].s[1] + e.s[1]) +
+                           std::fabs(s.csr[m].s[2] + e.s[2]);
+          if (best < 0 || d < best_d) {
+            best = m - s.csr_offsets[e.other];
+            best_d = d;
+          }
+        }
+        GSK_CHECK(best >= 0, "an internal face has no reverse entry in its neighbour's row");
+        adj_mate_host_[static_cast<std::size_t>(k)] = best;
+      }
+    for (int b = 0; b < nb_; ++b)
+      GSK_CHECK(adj_bslot_host_[static_cast<std::size_t>(b)] >= 0, "a boundary face is missing from its owner's row");
     std::vector<F> eps2(static_cast<std::size_t>(nc_));
     for (int c = 0; c < nc_; ++c) eps2[static_cast<std::size_t>(c)] = static_cast<F>(s.eps2[c]);
     eps2_.upload(eps2.data(), eps2.size(), stream_);
@@ -1360,6 +1404,8 @@ class HipBackend final : public Backend {
     }
     bf[static_cast<std::size_t>(nb_)].kind = dev::kBcPeriodic;
     bfaces_.upload(bf.data(), bf.size(), stream_);
+    host_bface_owner_.resize(static_cast<std::size_t>(nb_));
+    for (int b = 0; b < nb_; ++b) host_bface_owner_[static_cast<std::size_t>(b)] = s.bfaces[b].owner;
     host_bface_kind_.resize(static_cast<std::size_t>(nb_));
     for (int b = 0; b < nb_; ++b) host_bface_kind_[static_cast<std::size_t>(b)] = s.bfaces[b].kind;
     inflow_.alloc(static_cast<std::size_t>(nb_));
@@ -1898,6 +1944,130 @@ class HipBackend final : public Backend {
     lim_mode_ = mode;
   }
 
+  // ---- adjoint operators (kinetic_adjoint.hpp), FP64 ----
+  unsigned adj_grid(long long items) const { return static_cast<unsigned>(std::max(1LL, std::min(items, 1LL << 18))); }
+
+  void adjoint_prepare() override {
+    if constexpr (!std::is_same_v<F, double>) {
+      throw Error("adjoint operators need the FP64 form ([kinetic].precision = \"fp64\")");
+    } else {
+      GSK_CHECK(lim_mode_ == 2, "adjoint operators need the limiter branches recorded at the linearisation state");
+      const std::size_t nf = static_cast<std::size_t>(nc_) * static_cast<std::size_t>(nv_);
+      if (adj_x_[0].n == 0) {
+        std::size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        GSK_CHECK(8 * nf * sizeof(double) < free_b,
+                  strf("the adjoint workspace needs %.2f GB but only %.2f GB of device memory is free",
+                       8.0 * nf * sizeof(double) / 1e9, free_b / 1e9));
+        for (auto* b : {&adj_x_[0], &adj_x_[1], &adj_x_[2], &adj_o_[0], &adj_o_[1], &adj_o_[2], &adj_o_[3], &adj_o_[4]})
+          b->alloc(nf);
+        adj_lp_.alloc(static_cast<std::size_t>(nc_) * dev::kAdjPStride);
+        adj_jacp_.alloc(static_cast<std::size_t>(nc_) * dev::kAdjParams * dev::kMomentSums);
+        adj_jacm_.alloc(static_cast<std::size_t>(nc_) * dev::kMoments * dev::kMomentSums);
+        adj_lm_.alloc(static_cast<std::size_t>(nc_) * dev::kMoments);
+        adj_lb_.alloc(static_cast<std::size_t>(nb_) * 6 + 1);
+        adj_ls_.alloc(static_cast<std::size_t>(nb_) * dev::kBoundarySums + 1);
+        adj_shift_.alloc(static_cast<std::size_t>(nc_) * 3);
+        if (n_mw_ > 0) {
+          adj_lspec_.alloc(static_cast<std::size_t>(n_mw_) * nv_);
+          adj_lspec1_.alloc(static_cast<std::size_t>(n_mw_) * nv_);
+        }
+        adj_mate_.upload(adj_mate_host_.data(), adj_mate_host_.size(), stream_);
+        std::vector<int> slot(adj_bslot_host_), owners;
+        slot.push_back(0);
+        adj_bslot_.upload(slot.data(), slot.size(), stream_);
+        for (int b = 0; b < nb_; ++b) owners.push_back(host_bface_owner_[static_cast<std::size_t>(b)]);
+        std::sort(owners.begin(), owners.end());
+        owners.erase(std::unique(owners.begin(), owners.end()), owners.end());
+        n_adj_cells_ = static_cast<int>(owners.size());
+        owners.push_back(0);
+        adj_cells_.upload(owners.data(), owners.size(), stream_);
+      }
+      HIPCHK(hipMemcpyAsync(adj_shift_.p, shift_.p, adj_shift_.bytes(), hipMemcpyDeviceToDevice, stream_));
+      dev::k_adj_param_jacobian<<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), adj_shift_.p, volume_.p, vdt_.p, mu0_,
+                                                                          omega_, Pr_, 1e-5, adj_jacp_.p, adj_jacm_.p);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+
+  void adjoint_stage1(const void* lam, void* out, double* inflow_adj, std::size_t n) override {
+    if constexpr (!std::is_same_v<F, double>) {
+      throw Error("adjoint operators need the FP64 form");
+    } else {
+      GSK_CHECK(adj_x_[0].n != 0, "adjoint operators are not prepared");
+      GSK_CHECK(n == static_cast<std::size_t>(nb_) * 6, "inflow adjoint length differs from 6 x boundary faces");
+      const dev::View<double> v = view();
+      const double* l = static_cast<const double*>(lam);
+      double* o = static_cast<double*>(out);
+      const int tiles = (nv_ + dev::kAdjBlock - 1) / dev::kAdjBlock;
+      const unsigned grid = adj_grid(static_cast<long long>(nc_) * tiles);
+      if (n_mw_ > 0) {
+        HIPCHK(hipMemsetAsync(adj_lspec_.p, 0, adj_lspec_.bytes(), stream_));
+        HIPCHK(hipMemsetAsync(adj_lspec1_.p, 0, adj_lspec1_.bytes(), stream_));
+      }
+      dev::k_adj_transport<<<grid, dev::kAdjBlock, 0, stream_>>>(v, adj_mate_.p, l, o, adj_x_[0].p, adj_x_[1].p,
+                                                                  adj_x_[2].p, adj_lspec_.p, tiles);
+      dev::k_adj_collision<<<static_cast<unsigned>(std::min(nc_, 65535 * 4)), dev::kAdjBlock, 0, stream_>>>(v, l, adj_lp_.p);
+      if (n_mw_ > 0)
+        dev::k_adj_reflection<true><<<dim3((nv_ + 255) / 256, nb_), dev::kAdjBlock, 0, stream_>>>(
+            v, adj_lspec_.p, o, adj_x_[0].p, adj_x_[1].p, adj_x_[2].p, adj_lp_.p);
+      dev::k_adj_gradient<false><<<grid, dev::kAdjBlock, 0, stream_>>>(v, nullptr, 0, adj_x_[0].p, adj_x_[1].p,
+                                                                       adj_x_[2].p, adj_o_[0].p, adj_o_[1].p, adj_o_[2].p,
+                                                                       adj_o_[3].p, adj_o_[4].p, o, tiles);
+      if (nb_ > 0)
+        dev::k_adj_inflow<<<static_cast<unsigned>(std::min(nb_, 65535)), dev::kAdjBlock, 0, stream_>>>(
+            v, adj_bslot_.p, l, adj_o_[0].p, adj_o_[1].p, adj_o_[2].p, adj_o_[3].p, adj_o_[4].p, adj_lb_.p);
+      HIPCHK(hipGetLastError());
+      if (nb_ > 0) download(inflow_adj, adj_lb_.p, n);
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+
+  void adjoint_stage2(const double* sums_adj, std::size_t n_sums, const double* moments_adj, std::size_t n_m,
+                      void* out) override {
+    if constexpr (!std::is_same_v<F, double>) {
+      throw Error("adjoint operators need the FP64 form");
+    } else {
+      GSK_CHECK(adj_x_[0].n != 0, "adjoint operators are not prepared");
+      GSK_CHECK(n_sums == static_cast<std::size_t>(nb_) * dev::kBoundarySums,
+                "boundary-sum adjoint length differs from 20 x boundary faces");
+      GSK_CHECK(moments_adj == nullptr || n_m == static_cast<std::size_t>(nc_) * dev::kMoments,
+                "moment adjoint length differs from 15 x cells");
+      const dev::View<double> v = view();
+      double* o = static_cast<double*>(out);
+      const int tiles = (nv_ + dev::kAdjBlock - 1) / dev::kAdjBlock;
+      const unsigned grid = adj_grid(static_cast<long long>(nc_) * tiles);
+      if (nb_ > 0) HIPCHK(hipMemcpyAsync(adj_ls_.p, sums_adj, n_sums * sizeof(double), hipMemcpyHostToDevice, stream_));
+      if (moments_adj)
+        HIPCHK(hipMemcpyAsync(adj_lm_.p, moments_adj, n_m * sizeof(double), hipMemcpyHostToDevice, stream_));
+      for (auto* b : {&adj_x_[0], &adj_x_[1], &adj_x_[2]}) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes(), stream_));
+      if (n_mw_ > 0) {
+        HIPCHK(hipMemsetAsync(adj_lspec_.p, 0, adj_lspec_.bytes(), stream_));
+        HIPCHK(hipMemsetAsync(adj_lspec1_.p, 0, adj_lspec1_.bytes(), stream_));
+      }
+      if (nb_ > 0) {
+        dev::k_adj_sums<<<static_cast<unsigned>(std::min(nb_, 65535)), dev::kAdjBlock, 0, stream_>>>(
+            v, adj_ls_.p, o, adj_x_[0].p, adj_x_[1].p, adj_x_[2].p, adj_lspec_.p);
+        if (n_mw_ > 0)
+          dev::k_adj_reflection<true><<<dim3((nv_ + 255) / 256, nb_), dev::kAdjBlock, 0, stream_>>>(
+              v, adj_lspec_.p, o, adj_x_[0].p, adj_x_[1].p, adj_x_[2].p, adj_lp_.p);
+        dev::k_adj_gradient<true><<<adj_grid(static_cast<long long>(n_adj_cells_) * tiles), dev::kAdjBlock, 0, stream_>>>(
+            v, adj_cells_.p, n_adj_cells_, adj_x_[0].p, adj_x_[1].p, adj_x_[2].p, adj_o_[0].p, adj_o_[1].p, adj_o_[2].p,
+            adj_o_[3].p, adj_o_[4].p, o, tiles);
+      }
+      dev::k_adj_gather<<<grid, dev::kAdjBlock, 0, stream_>>>(v, adj_mate_.p, adj_o_[0].p, adj_o_[1].p, adj_o_[2].p,
+                                                               adj_o_[3].p, adj_o_[4].p, o, adj_lspec1_.p, tiles);
+      if (n_mw_ > 0)
+        dev::k_adj_reflection<false><<<dim3((nv_ + 255) / 256, nb_), dev::kAdjBlock, 0, stream_>>>(
+            v, adj_lspec1_.p, o, adj_x_[0].p, adj_x_[1].p, adj_x_[2].p, adj_lp_.p);
+      dev::k_adj_moments<<<static_cast<unsigned>(std::min(nc_, 65535 * 4)), dev::kAdjBlock, 0, stream_>>>(
+          v, adj_shift_.p, adj_jacp_.p, adj_jacm_.p, adj_lp_.p, moments_adj ? adj_lm_.p : nullptr, o);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(stream_));
+    }
+  }
+
   void precondition_device(const void* r, void* out) override {
     const F* ri = static_cast<const F*>(r);
     F* o = static_cast<F*>(out);
@@ -2289,6 +2459,12 @@ class HipBackend final : public Backend {
   int grid_n_[3] = {0, 0, 0};
   double mu0_ = 0, omega_ = 0, Pr_ = 0, cold_bgk_ = 0, diag_scale_ = 1, interface_damping_ = 0, weight_ = 0;
   int discrete_eq_ = 0, deq_fallback_ = 0, limiter_mode_ = 1, positivity_ = 0;
+  // adjoint workspace (adjoint_prepare)
+  DBuf<double> adj_x_[3], adj_o_[5], adj_lp_, adj_jacp_, adj_jacm_, adj_lm_, adj_lb_, adj_ls_, adj_shift_, adj_lspec_,
+      adj_lspec1_;
+  DBuf<int> adj_mate_, adj_bslot_, adj_cells_;
+  std::vector<int> adj_mate_host_, adj_bslot_host_, host_bface_owner_;
+  int n_adj_cells_ = 0;
   int lim_mode_ = 0, max_faces_ = 0;  // limiter branches (set_limiter_branches); most faces of a cell
   DBuf<unsigned> branch_;
   dev::DeqGrid deq_{};

@@ -196,8 +196,54 @@ class SteadyProblem:
             return (Rp - Rm) / (2.0 * h), (mp - mm) / (2.0 * h)
         raise ValueError(which)
 
-    def _gmres(self, b: torch.Tensor, tol: float, restart: int, max_iterations: int):
-        """Right-preconditioned restarted GMRES for J s = b; returns s, iterations, relative residual."""
+    # -- adjoint operators ----------------------------------------------------------------------
+    def _reversed_velocities(self) -> torch.Tensor:
+        """Storage position of -xi for every velocity position. The first-order transport of velocity
+        xi, transposed, is the transport of -xi (each face coefficient xi . S changes sign, and a
+        closed cell's outflow sum equals its inflow sum), so LU-SGS applied to the velocity-reversed
+        vector, reversed back, is to J^T what LU-SGS is to J: the same level-scheduled sweeps, run
+        for the opposite direction, i.e. in the reversed level order."""
+        if getattr(self, "_reverse", None) is None:
+            v = self.sv.setup.view()
+            nv = self.sv.setup.nv
+            xi = np.stack([np.ctypeslib.as_array(a, shape=(nv,)) for a in (v.xi_x, v.xi_y, v.xi_z)], axis=1)
+            scale = 1e9 / max(float(np.abs(xi).max()), 1e-300)
+            keys = {tuple(k): p for p, k in enumerate(np.rint(xi * scale).astype(np.int64))}
+            rev = [keys.get(tuple(k)) for k in np.rint(-xi * scale).astype(np.int64)]
+            if any(r is None for r in rev):
+                raise ValueError("the velocity grid is not symmetric under xi -> -xi: the adjoint solve needs one")
+            self._reverse = torch.tensor(rev, dtype=torch.int64, device="cuda")
+        return self._reverse
+
+    def _reverse_velocities(self, x: torch.Tensor) -> torch.Tensor:
+        nc = self.sv.setup.n_cells
+        return x.view(nc, -1)[:, self._reversed_velocities()].reshape(-1)
+
+    def _precondition_t(self, r: torch.Tensor) -> torch.Tensor:
+        """~ J^-T r: LU-SGS on the velocity-reversed r, reversed back."""
+        return self._reverse_velocities(self._precondition(self._reverse_velocities(r).contiguous()))
+
+    def _jtv(self, lam: torch.Tensor) -> torch.Tensor:
+        """J^T lam at the prepared state (gsk_solver_adjoint_apply_device)."""
+        out = self._empty()
+        torch.cuda.synchronize()
+        self.sv.adjoint_apply_device(lam.data_ptr(), [0.0] * self.sv.setup.n_patches, out.data_ptr(), self.n)
+        return out
+
+    def _mdot_gradient(self) -> torch.Tensor:
+        """(dMdot/df)^T at the prepared state: the transposed boundary model with the inlet's weight."""
+        w = [0.0] * self.sv.setup.n_patches
+        w[self.inlet] = -self.factor
+        out = self._empty()
+        torch.cuda.synchronize()
+        self.sv.adjoint_apply_device(0, w, out.data_ptr(), self.n)
+        return out
+
+    def _gmres(self, b: torch.Tensor, tol: float, restart: int, max_iterations: int, op=None, prec=None):
+        """Right-preconditioned restarted GMRES for A s = b (A = J and LU-SGS unless op / prec are
+        given); returns s, iterations, relative residual."""
+        op = op or (lambda z: self._jv(z)[0])
+        prec = prec or self._precondition
         bnorm = float(torch.linalg.vector_norm(b))
         s = torch.zeros_like(b)
         if bnorm == 0.0:
@@ -222,7 +268,7 @@ class SteadyProblem:
             g[0] = beta
             k_used = 0
             for k in range(m):
-                w, _ = self._jv(self._precondition(V[k]))
+                w = op(prec(V[k]))
                 for _ in range(2):  # classical Gram-Schmidt, twice
                     h = torch.stack([torch.dot(V[i], w) for i in range(k + 1)]).double().cpu().numpy()
                     for i in range(k + 1):
@@ -251,15 +297,17 @@ class SteadyProblem:
             z = torch.zeros_like(s)
             for i in range(k_used):
                 z.add_(V[i], alpha=float(y[i]))
-            s = s + self._precondition(z)
+            s = s + prec(z)
             del V, z
-            Js, _ = self._jv(s)
-            r = b - Js
+            r = b - op(s)
         return s, total, rel
 
-    def _bicgstab(self, b: torch.Tensor, tol: float, max_iterations: int):
-        """Right-preconditioned BiCGStab for J s = b (short recurrences: a few vectors of state,
-        two products with J and two preconditioner passes per iteration)."""
+    def _bicgstab(self, b: torch.Tensor, tol: float, max_iterations: int, op=None, prec=None):
+        """Right-preconditioned BiCGStab for A s = b (A = J and LU-SGS unless op / prec are given;
+        short recurrences: a few vectors of state, two products with A and two preconditioner passes
+        per iteration)."""
+        op = op or (lambda z: self._jv(z)[0])
+        prec = prec or self._precondition
         bnorm = float(torch.linalg.vector_norm(b))
         x = torch.zeros_like(b)
         if bnorm == 0.0:
@@ -277,16 +325,16 @@ class SteadyProblem:
                 break
             beta = (rho_new / rho) * (alpha / omega)
             p = r + beta * (p - omega * v)
-            y = self._precondition(p)
-            v, _ = self._jv(y)
+            y = prec(p)
+            v = op(y)
             alpha = rho_new / float(torch.dot(rhat, v))
             x = x + alpha * y
             r = r - alpha * v
             rel = float(torch.linalg.vector_norm(r)) / bnorm
             if rel <= tol:
                 break
-            z = self._precondition(r)
-            t, _ = self._jv(z)
+            z = prec(r)
+            t = op(z)
             tt = float(torch.dot(t, t))
             omega = float(torch.dot(t, r)) / tt if tt > 0.0 else 0.0
             x = x + omega * z
@@ -346,10 +394,58 @@ class SteadyProblem:
         return out
 
 
+    def adjoint_sensitivities(self, params=("delta_rp", "alpha"), tol: float = 1e-8, restart: int = 30,
+                              max_iterations: int = 400, polish: int = 0) -> dict:
+        """dMdot/dp at the converged state for each named parameter from one adjoint solve:
+        J^T nu = (dMdot/df)^T, then dMdot/dp = dMdot/dp|_f - nu . dR/dp. J^T is the hand-written
+        transposed Jacobian on the limiter branches of f* (gsk_solver_adjoint_apply_device), the
+        solve BiCGStab right-preconditioned by LU-SGS of the reversed velocities, with optional
+        restarted-GMRES polishing on the true residual."""
+        t0 = time.perf_counter()
+        if "alpha" in params and self.wall_patch is None:
+            raise ValueError("dMdot/dalpha needs the Maxwell-wall patch (wall_patch)")
+        self.sv.set_newton_tolerances(0.0, 0.0)
+        self.sv.set_boundary_iterations(10)
+        out = {}
+        try:
+            self.sv.freeze_limiters(True)  # recorded by the residual evaluation at f* in _capture
+            self._capture()
+            derivs = {p: self._param_derivative(p) for p in params}
+            self._residual(self.f0)  # back at f* (cell parameters and boundary state of f*)
+            self.sv.adjoint_prepare()
+            g = self._mdot_gradient()
+            gn = float(torch.linalg.vector_norm(g))
+            nu, its, rel = self._bicgstab(g, tol, max_iterations, op=self._jtv, prec=self._precondition_t)
+            r_true = g - self._jtv(nu)
+            true_rel = float(torch.linalg.vector_norm(r_true)) / gn if gn > 0.0 else 0.0
+            if polish > 0 and true_rel > tol:
+                d, k, _ = self._gmres(r_true, tol * gn / float(torch.linalg.vector_norm(r_true)), restart, polish,
+                                      op=self._jtv, prec=self._precondition_t)
+                nu = nu + d
+                its += k
+                true_rel = float(torch.linalg.vector_norm(g - self._jtv(nu))) / gn
+            if self.log is not None:
+                self.log(f"adjoint: {its} Krylov iterations, true relative residual {true_rel:.2e}")
+            for p, (dR, dM) in derivs.items():
+                out[p] = dM - float(torch.dot(nu, dR))
+            self.stats["adjoint_krylov_iterations"] = its
+            self.stats["adjoint_krylov_relative_residual"] = rel
+            self.stats["adjoint_true_relative_residual"] = true_rel
+        finally:
+            self.sv.freeze_limiters(False)
+            self.sv.set_newton_tolerances(*self.newton_tolerances)
+            self.sv.set_boundary_iterations(self.boundary_iterations)
+            if self.f0 is not None:
+                self._set_f(self.f0)  # leave the solver at f*
+                self.sv.prepare()
+        self.stats["sensitivity_seconds"] = time.perf_counter() - t0
+        return out
+
+
 class ConvergedMdot(torch.autograd.Function):
     """Mdot of the converged kinetic solve as a function of (delta_rp, alpha); backward returns the
-    sensitivities from SteadyProblem.sensitivities (branch-frozen difference Jacobian-vector products,
-    LU-SGS right preconditioning, BiCGStab with optional restarted-GMRES polishing)."""
+    sensitivities from one adjoint solve (SteadyProblem.adjoint_sensitivities; problem.method =
+    "tangent" uses one tangent solve per parameter instead, SteadyProblem.sensitivities)."""
 
     @staticmethod
     def forward(ctx, delta_rp: torch.Tensor, alpha: torch.Tensor, problem: SteadyProblem, residual_target: float):
@@ -360,8 +456,12 @@ class ConvergedMdot(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad):
-        params = ("delta_rp", "alpha") if ctx.problem.wall_patch is not None else ("delta_rp",)
-        sens = ctx.problem.sensitivities(params=params, polish=getattr(ctx.problem, "polish", 0))
+        prob = ctx.problem
+        params = ("delta_rp", "alpha") if prob.wall_patch is not None else ("delta_rp",)
+        if getattr(prob, "method", "adjoint") == "tangent":
+            sens = prob.sensitivities(params=params, polish=getattr(prob, "polish", 0))
+        else:
+            sens = prob.adjoint_sensitivities(params=params, polish=getattr(prob, "polish", 0))
         g = float(grad)
         return (torch.tensor(g * sens["delta_rp"], dtype=torch.float64),
                 torch.tensor(g * sens.get("alpha", 0.0), dtype=torch.float64), None, None)

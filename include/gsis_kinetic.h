@@ -302,6 +302,17 @@ int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n);
  * differences of it give the Jacobian products of that branch; on = 0 restores the normal limiter.
  * Costs 4 bytes per cell and velocity of device memory while on. */
 int gsk_solver_freeze_limiters(gsk_solver* sv, int32_t on);
+/* Adjoint operators (no counterpart in the reference, which has no derivative capability; for
+ * sensitivity solves, FP64 solvers only). gsk_solver_adjoint_prepare: at the current f, whose limiter
+ * branches gsk_solver_freeze_limiters has recorded there, tabulates what the transposed Jacobian of
+ * the steady residual needs (Jacobians of the boundary model's face maps and of the cell parameters);
+ * closed domains and mass-flow inlets are refused. gsk_solver_adjoint_apply_device:
+ * out = J^T lam + (d/df sum_k patch_weights[k] patch_mass_flow[k])^T at that state (n_patches
+ * weights; lam, out device arrays [cell][storage position], lam may be NULL for 0); fails once the
+ * state has changed since the prepare. Needs 8 doubles per cell and velocity of device workspace. */
+int gsk_solver_adjoint_prepare(gsk_solver* sv);
+int gsk_solver_adjoint_apply_device(gsk_solver* sv, const void* lam, const double* patch_weights, size_t n_patches,
+                                    void* out, size_t n);
 gsk_backend* gsk_solver_backend(gsk_solver* sv);
 
 #ifdef __cplusplus
