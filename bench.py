@@ -250,7 +250,7 @@ def time_to_mdot_runs(g, mg, forms, mesh="40k", delta=10.0, uniform_grid=False, 
 
 
 def time_to_mdot(g, mg, args):
-    out = time_to_mdot_runs(g, mg, ["fp64", "fp32", "mixed"], mesh=args.ttm_mesh, delta=args.ttm_delta,
+    out = time_to_mdot_runs(g, mg, args.ttm_forms.split(","), mesh=args.ttm_mesh, delta=args.ttm_delta,
                             uniform_grid=args.uniform_grid, residual_target=args.residual_target,
                             max_iterations=args.max_iterations)
     print(json.dumps(out), flush=True)
@@ -259,9 +259,12 @@ def time_to_mdot(g, mg, args):
 def sensitivity_bench(g, mg, args):
     """Metric 3: cost of dMdot/d delta_rp and dMdot/d alpha (sensitivity solve) relative to one
     converged forward solve, on the 40k-cell tube with a Maxwell wall (24^3, delta_rp = 10), with
-    central differences of re-converged FP64 solves as the check. --sens-forward mixed runs the
-    forward solve in the mixed form (FP32 f, matrix-core moment sums) and the sensitivity solve in
-    FP64 at that state; fp64 runs both in FP64 (the FP64 reference gradient)."""
+    central differences of re-converged FP64 solves as the check. --sens-method adjoint (default):
+    one adjoint solve with the hand-written transposed Jacobian; tangent: one tangent solve per
+    parameter with branch-frozen difference products. --sens-forward mixed runs the forward solve in
+    the mixed form (FP32 f, matrix-core moment sums) and the sensitivity solve in FP64 at that state;
+    fp64 runs both in FP64 (the FP64 reference gradient). The device-memory high-water mark is the
+    largest total - free seen at the sensitivity solve's operator applications."""
     import gsis_sim_amd.sensitivity as sens
     import torch
 
@@ -296,13 +299,30 @@ def sensitivity_bench(g, mg, args):
         fwd_s, fwd_its = prob.stats["forward_seconds"], prob.stats["forward_iterations"]
         if sv32 is not None:
             sv32.close()
-        log(f"forward: {fwd_its} iterations, {fwd_s:.1f} s, Mdot {float(mdot.detach()):.10f}; sensitivity solve")
+        log(f"forward: {fwd_its} iterations, {fwd_s:.1f} s, Mdot {float(mdot.detach()):.10f}; "
+            f"{args.sens_method} sensitivity solve")
         prob.polish = args.polish
+        prob.method = args.sens_method
+        prob.restart = args.restart
+        high = [0]
+
+        def sampled(fn):
+            def run(*a):
+                free, total = torch.cuda.mem_get_info()
+                high[0] = max(high[0], total - free)
+                return fn(*a)
+            return run
+        prob._jtv = sampled(prob._jtv)
+        prob._precondition = sampled(prob._precondition)
         mdot.backward()
         out = {"metric": "sensitivity / forward cost",
-               "workload": f"tube40k_24cubed_maxwell_alpha{alpha:g}_{args.sens_forward}_forward",
+               "workload": f"tube40k_24cubed_maxwell_alpha{alpha:g}_{args.sens_forward}_forward_{args.sens_method}",
                "forward_form": "mixed (FP32 f, matrix-core moment sums)" if mixed else "fp64",
-               "sensitivity_form": "fp64, limiter branches frozen at the forward state",
+               "sensitivity_method": args.sens_method,
+               "sensitivity_form": ("fp64 adjoint: hand-written transposed Jacobian on the limiter branches of the "
+                                    "forward state, one solve for both parameters" if args.sens_method == "adjoint" else
+                                    "fp64 tangent: branch-frozen difference Jacobian products, one solve per parameter"),
+               "device_memory_high_water_gb": high[0] / 1e9,
                "delta_rp": delta, "alpha": alpha, "sens_eps": args.sens_eps, "mdot": float(mdot.detach()),
                "forward_seconds": fwd_s, "forward_iterations": fwd_its, "forward_residual_target": target,
                "sensitivity_seconds": prob.stats["sensitivity_seconds"],
@@ -348,6 +368,7 @@ def main():
     ap.add_argument("--ttm-mesh", choices=["40k", "135k"], default="40k",
                     help="time-to-Mdot tube: 40k cells with 24^3 velocities, or 135k cells with 32^3")
     ap.add_argument("--ttm-delta", type=float, default=10.0, help="time-to-Mdot rarefaction parameter delta_rp")
+    ap.add_argument("--ttm-forms", default="fp64,fp32,mixed", help="time-to-Mdot forms (fp64 first)")
     ap.add_argument("--sensitivity", action="store_true",
                     help="cost of dMdot/d delta_rp and dMdot/d alpha relative to one converged forward solve "
                          "(40k tube with a Maxwell wall, FP64)")
@@ -360,6 +381,9 @@ def main():
                     help="forward residual target of the mixed form for --sens-forward mixed")
     ap.add_argument("--fd-residual", type=float, default=1e-12, help="residual target of the difference solves")
     ap.add_argument("--polish", type=int, default=120, help="GMRES iterations polishing the BiCGStab solution")
+    ap.add_argument("--restart", type=int, default=20, help="GMRES restart length of the polishing (vectors held)")
+    ap.add_argument("--sens-method", choices=["adjoint", "tangent"], default="adjoint",
+                    help="with --sensitivity: one adjoint solve, or one tangent solve per parameter")
     ap.add_argument("--sens-eps", type=float, default=1e-5,
                     help="relative perturbation of the difference Jacobian-vector products")
     ap.add_argument("--fd-params", default="delta_rp,alpha", help="parameters the differences check")

@@ -2,15 +2,17 @@
 a Maxwell wall's accommodation alpha, behind a torch.autograd.Function (no counterpart in the
 reference, which has no derivative capability).
 
-At the converged state f* the steady residual vanishes, R(f*; p) = 0, so
-    dMdot/dp = dMdot/dp|_f + (dMdot/df) s,    J s = -dR/dp,    J = dR/df at f*.
+At the converged state f* the steady residual vanishes, R(f*; p) = 0, so with J = dR/df at f*
+    adjoint (default):  J^T nu = (dMdot/df)^T,  dMdot/dp = dMdot/dp|_f - nu . dR/dp   (one solve)
+    tangent:            J s = -dR/dp,           dMdot/dp = dMdot/dp|_f + (dMdot/df) s  (per parameter)
 R is the solver's full steady residual (transport with the limited reconstruction, Shakhov
 collision with the discrete equilibrium, every boundary condition, wall densities and inflow states
-recomputed from f), evaluated on the device through gsk_solver_residual_device. This is a forward
-(tangent) solve per parameter; with two parameters it costs what one adjoint solve and an extra
-right-hand side would, and it needs no transposed operator. J s = -dR/dp is solved by BiCGStab (then
-restarted GMRES on the true residual) with right preconditioning by the LU-SGS passes of one inner
-sweep (gsk_solver_precondition_device, the level-scheduled kernels of the forward solve).
+recomputed from f), evaluated on the device through gsk_solver_residual_device. The adjoint applies
+the hand-written transposed Jacobian (gsk_solver_adjoint_apply_device, which also gives
+(dMdot/df)^T) and is preconditioned by LU-SGS of the reversed velocities (the transposed first-order
+transport of xi is the transport of -xi); the tangent solve takes difference Jacobian-vector
+products and LU-SGS (gsk_solver_precondition_device, the level-scheduled kernels of the forward
+solve). Both solve by BiCGStab, then restarted GMRES on the true residual.
 
 Jacobian-vector products. R is only piecewise smooth in f: every limited gradient takes the largest
 and the most negative face projection, the extreme neighbour values and the smaller of two limiter
@@ -26,8 +28,10 @@ so R does not jump with the iteration count) and runs 10 inflow / boundary-sum r
 (the pressure-inlet state depends on the previous round's face states), and restores the case's
 settings afterwards.
 
-delta_rp enters only through the nondimensional viscosity scale mu0 = 1 / (sqrt(2) delta_rp); alpha
-only through the specular fraction 1 - alpha of the Maxwell-wall faces.
+The adjoint transposes R on the same recorded branches, so the two forms differentiate the same
+function (they agree to ~1e-9). dR/dp is a central difference in the parameter in both. delta_rp
+enters only through the nondimensional viscosity scale mu0 = 1 / (sqrt(2) delta_rp); alpha only
+through the specular fraction 1 - alpha of the Maxwell-wall faces.
 """
 
 from __future__ import annotations
@@ -416,6 +420,7 @@ class SteadyProblem:
             g = self._mdot_gradient()
             gn = float(torch.linalg.vector_norm(g))
             nu, its, rel = self._bicgstab(g, tol, max_iterations, op=self._jtv, prec=self._precondition_t)
+            torch.cuda.empty_cache()  # the BiCGStab vectors, before GMRES builds its basis
             r_true = g - self._jtv(nu)
             true_rel = float(torch.linalg.vector_norm(r_true)) / gn if gn > 0.0 else 0.0
             if polish > 0 and true_rel > tol:
@@ -458,10 +463,11 @@ class ConvergedMdot(torch.autograd.Function):
     def backward(ctx, grad):
         prob = ctx.problem
         params = ("delta_rp", "alpha") if prob.wall_patch is not None else ("delta_rp",)
+        opts = dict(params=params, polish=getattr(prob, "polish", 0), restart=getattr(prob, "restart", 30))
         if getattr(prob, "method", "adjoint") == "tangent":
-            sens = prob.sensitivities(params=params, polish=getattr(prob, "polish", 0))
+            sens = prob.sensitivities(**opts)
         else:
-            sens = prob.adjoint_sensitivities(params=params, polish=getattr(prob, "polish", 0))
+            sens = prob.adjoint_sensitivities(**opts)
         g = float(grad)
         return (torch.tensor(g * sens["delta_rp"], dtype=torch.float64),
                 torch.tensor(g * sens.get("alpha", 0.0), dtype=torch.float64), None, None)

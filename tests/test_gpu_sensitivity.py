@@ -7,8 +7,9 @@
   Jacobian products on those branches are linear in the direction (J(a z1 + b z2) = a J z1 + b J z2)
   and independent of the difference step to 1e-7 relative; the same products of the plain residual
   are not (they cross limiter branches).
-* dMdot/d delta_rp and dMdot/d alpha against central differences (step 1e-4, relative in delta_rp,
-  absolute in alpha) of re-converged solves, within 1e-4 relative, on a tube with a curved Maxwell
+* dMdot/d delta_rp and dMdot/d alpha from the tangent solves (one per parameter) against central
+  differences (step 1e-4, relative in delta_rp, absolute in alpha) of re-converged solves, within
+  1e-4 relative, on a tube with a curved Maxwell
   wall at alpha = 0.8, 0.9 and 1.0 (1.0 through the wall's smooth extension: the case is a Maxwell
   wall set to alpha = 1).
 """
@@ -104,6 +105,7 @@ def test_sensitivities_match_central_differences(gsk, meshgen, tmp_path, alpha):
     sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
     mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=1e-12)
     prob.polish = 60
+    prob.method = "tangent"  # one tangent solve per parameter (the adjoint has its own tests)
     mdot.backward()
     g_delta, g_alpha = float(d_t.grad), float(a_t.grad)
     print(f"  Mdot {float(mdot.detach()):.10f} dMdot/ddelta {g_delta:.8e} dMdot/dalpha {g_alpha:.8e} {prob.stats}",
