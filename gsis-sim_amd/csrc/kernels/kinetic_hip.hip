@@ -62,6 +62,16 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
   __syncthreads();
 }
 
+// fixed-order wave reduction: an xor butterfly, after which every lane holds the same sum (each step
+// adds a pair of values, and a + b = b + a, so the two lanes of a pair agree)
+template <int N>
+__device__ __forceinline__ void wave_sum(double (&v)[N]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += __shfl_xor(v[k], off, 64);
+}
+
 template <class F, bool Max>
 __device__ __forceinline__ F block_extreme(F v, F* lds) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -172,6 +182,85 @@ __device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F*
 #pragma unroll
         for (int k = 0; k < 5; ++k) gamma[k] = lds[k];
         __syncthreads();
+      }
+      return it + 1;
+    }
+    if (r < 0) break;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) beta[k] = b[k];
+    rel_prev = rel;
+  }
+  out = maxw_from_state<F>(rho, ux, uy, uz, T);
+  for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+  return -1;
+}
+
+// The same discrete Maxwellian with one wave per cell: the wave's lanes share the velocity sums
+// (wave_sum leaves the total in every lane), and every lane takes the Newton step itself, so no lane
+// waits on another and the workgroup's other waves (other cells) fill the latency of the step. Same
+// iteration, step control, accept rule and result as block_discrete_maxwell up to the order of the
+// sums.
+template <class F>
+__device__ int wave_discrete_maxwell(int nv, const F* __restrict__ xs, const F* __restrict__ ys,
+                                     const F* __restrict__ zs, double weight, const F* __restrict__ wq,
+                                     const DeqGrid& grid, double tolerance, double rho, double ux, double uy, double uz,
+                                     double T, const Target<F>* poly, Maxw<F>& out, double (&gamma)[5]) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
+  out = maxw_from_state<F>(rho, ux, uy, uz, T);
+  if (deq_resolved(grid, ux, uy, uz, T)) return 0;
+  double beta[5];
+  deq_initial_beta(rho, T, beta);
+  const F fux = static_cast<F>(ux), fuy = static_cast<F>(uy), fuz = static_cast<F>(uz);
+  double rel_prev = 1e300;
+  for (int it = 0; it < grid.max_iterations; ++it) {
+    const Maxw<F> m = deq_params<F>(beta, ux, uy, uz);
+    double sums[kDeqSums];
+#pragma unroll
+    for (int k = 0; k < kDeqSums; ++k) sums[k] = 0.0;
+#pragma unroll 1
+    for (int p = lane; p < nv; p += 64) {
+      const F x = xs[p], y = ys[p], z = zs[p];
+      F t[kDeqSums];
+      F e = maxw_eval(m, x, y, z);
+      if (wq != nullptr) e *= wq[p];  // stretched grid: weight * wq[p] per velocity
+      deq_terms<F>(e, F(0), x - fux, y - fuy, z - fuz, t);
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] += static_cast<double>(t[k]);
+    }
+    {
+      double s15[kDeqNewtonSums];
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) s15[k] = sums[k];
+      wave_sum<kDeqNewtonSums>(s15);
+#pragma unroll
+      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] = s15[k];
+    }
+    double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
+    double rel = 0.0;
+    const int r = deq_newton_step(sums, weight, rho, T, tolerance, b, rel);
+    if (deq_accept(r, rel, rel_prev, it + 1 == grid.max_iterations)) {
+      out = m;
+      if (poly != nullptr) {
+        double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int p = lane; p < nv; p += 64) {
+          const F x = xs[p], y = ys[p], z = zs[p];
+          const F cx = x - fux, cy = y - fuy, cz = z - fuz;
+          const F pm = shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz);
+          F t[kDeqSums];
+          F e = maxw_eval(m, x, y, z);
+          if (wq != nullptr) e *= wq[p];
+          deq_terms<F>(e, pm, cx, cy, cz, t);
+#pragma unroll
+          for (int k = 0; k < 5; ++k) s5[k] += static_cast<double>(t[kDeqNewtonSums + k]);
+        }
+        wave_sum<5>(s5);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) sums[kDeqNewtonSums + k] = s5[k];
+        double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!deq_shakhov_correction(sums, weight, g))
+          for (int k = 0; k < 5; ++k) g[k] = 0.0;
+        for (int k = 0; k < 5; ++k) gamma[k] = g[k];
       }
       return it + 1;
     }
@@ -398,8 +487,6 @@ __global__ void k_scale_moments(double* __restrict__ moments, int n_cells, doubl
   for (int k = 5; k < kMoments; ++k) m[k] *= s;
 }
 
-// Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
-// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq)
 // Cell parameters of one cell from its 15 moments, all threads of the block cooperating on the
 // discrete-equilibrium sums; r: the Newton's iterations (0 resolved, -1 failed)
 template <class F>
@@ -428,18 +515,36 @@ __device__ CellP<F> cell_params_block(const View<F>& v, const double* m, double 
 }
 
 // Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
-// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq)
+// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq). One wave per cell
+// (wave_discrete_maxwell): kWaves cells per workgroup, no workgroup barrier.
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double* __restrict__ moments,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_cell_params(View<F> v, const double* __restrict__ moments,
                                                         const double* __restrict__ volume, const double* __restrict__ vdt,
                                                         double mu0, double omega, double Pr, CellP<F>* __restrict__ cells,
                                                         int* __restrict__ flags) {
-  __shared__ double lds[kWaves * kDeqSums];
-  for (int c = blockIdx.x; c < v.n_cells; c += gridDim.x) {
+  const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+  for (int c = blockIdx.x * kWaves + wave; c < v.n_cells; c += gridDim.x * kWaves) {
+    const double* m = moments + static_cast<i64>(c) * kMoments;
+    CellP<F> cp = make_cell_params<F>(m, volume[c], vdt[c], mu0, omega, Pr, v.cold_bgk);
     int r = 0;
-    const CellP<F> cp =
-        cell_params_block(v, moments + static_cast<i64>(c) * kMoments, volume[c], vdt[c], mu0, omega, Pr, r, lds);
-    if (threadIdx.x == 0) {
+    if (v.discrete_eq) {
+      Maxw<F> eq;
+      double gamma[5];
+      r = wave_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
+                                   m[2], m[3], m[4], &cp.s, eq, gamma);
+      if (r > 0) {
+        cp.s.eq = eq;
+        cp.s.g0 = static_cast<F>(gamma[0]);
+        cp.s.gx = static_cast<F>(gamma[1]);
+        cp.s.gy = static_cast<F>(gamma[2]);
+        cp.s.gz = static_cast<F>(gamma[3]);
+        cp.s.g4 = static_cast<F>(gamma[4]);
+      } else if (r < 0 && v.deq_fallback) {
+        cp.coll = F(0);
+        cp.diag = static_cast<F>(vdt[c]);
+      }
+    }
+    if (lane == 0) {
       cells[c] = cp;
       flags[c] = r;
     }
@@ -1630,7 +1735,8 @@ class HipBackend final : public Backend {
 
   void update_cell_params() override {
     begin(kOther);
-    dev::k_cell_params<F><<<grid_cells(), dev::kBlock, 0, stream_>>>(view(), moments_.p, volume_.p, vdt_.p, mu0_, omega_, Pr_,
+    dev::k_cell_params<F><<<static_cast<unsigned>(std::min((nc_ + dev::kWaves - 1) / dev::kWaves, 1 << 20)), dev::kBlock, 0,
+                            stream_>>>(view(), moments_.p, volume_.p, vdt_.p, mu0_, omega_, Pr_,
                                                                      cells_.p, flags_.p);
     for (DBuf<dev::SweepRec<F>>* recs : {&fw_recs_, &bw_recs_}) {
       const int nr = static_cast<int>(recs->n);
