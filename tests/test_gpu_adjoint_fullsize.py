@@ -1,0 +1,54 @@
+"""The adjoint sensitivities at the sensitivity configuration's size: the ~40k-cell tube (40,320
+cells, 24^3 uniform velocities on [-6, 6]^3, delta_rp = 10, FP64) with a Maxwell wall at
+alpha = 0.8. The forward solve converges to residual 1e-9, one adjoint solve gives dMdot/d delta_rp
+and dMdot/d alpha, and both must lie within 1e-4 relative of central differences (step 1e-4,
+relative in delta_rp, absolute in alpha) of re-converged solves at residual 1e-12, and the adjoint
+must cost at most 3x the forward solve. The four difference solves take about 7 minutes, so their
+values are constants here, from the run recorded in
+profiles/sensitivity_tube40k_24cubed_maxwell_alpha0.8_adjoint.json (bench.py --sensitivity
+--alpha 0.8 --fd-check); that run measured 1.2e-6 and 4.9e-7 and a cost of 0.78x.
+"""
+
+from __future__ import annotations
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+FD_DDELTA = 0.01899277790812226
+FD_DALPHA = -0.44044936697851256
+MDOT = 0.6974269243754333  # the forward solve's Mdot in that run
+
+
+@pytest.mark.timeout(600)
+def test_tube40k_alpha08_adjoint_within_1e4_of_central_differences(gsk, meshgen, tmp_path):
+    import gsis_sim_amd.sensitivity as sens
+
+    delta, alpha = 10.0, 0.8
+    path = meshgen.tube_case(str(tmp_path), n_core=16, n_ring=10, n_axial=45, n_vel=24, vrange=6.0, delta=delta,
+                             accommodation=alpha, kinetic={"precision": "fp64"}, name="tube_maxwell")
+    setup = gsk.Setup(path)
+    sv = gsk.Solver(setup)
+    prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
+                              alpha=alpha)
+    prob.log = lambda m: print(f"[gpu] tube40k alpha 0.8 {m}", flush=True)
+    prob.polish = 120
+    prob.restart = 20
+    d_t = torch.tensor(delta, dtype=torch.float64, requires_grad=True)
+    a_t = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=1e-9)
+    fwd = prob.stats["forward_seconds"]
+    mdot.backward()
+    cost = prob.stats["sensitivity_seconds"] / fwd
+    g_d, g_a = float(d_t.grad), float(a_t.grad)
+    e_d, e_a = abs(g_d - FD_DDELTA) / abs(FD_DDELTA), abs(g_a - FD_DALPHA) / abs(FD_DALPHA)
+    print(f"[gpu] tube40k alpha 0.8: Mdot {float(mdot.detach()):.10f} (recorded {MDOT:.10f}); dMdot/ddelta {g_d:.8e} "
+          f"({e_d:.1e}), dMdot/dalpha {g_a:.8e} ({e_a:.1e}); adjoint {prob.stats['sensitivity_seconds']:.1f} s = "
+          f"{cost:.2f} x forward", flush=True)
+    sv.close()
+    assert abs(float(mdot.detach()) - MDOT) <= 1e-8 * MDOT
+    assert e_d <= 1e-4 and e_a <= 1e-4
+    assert prob.stats["adjoint_true_relative_residual"] <= 1e-7
+    assert cost <= 3.0
