@@ -704,34 +704,10 @@ __device__ __forceinline__ bool xcd_tile(unsigned L, int n_cells, int tiles, int
   return true;
 }
 
-// The same with compact cell groups (View::group_cells: n_groups rows of group_size cells, -1 padding;
-// each group a connected piece of an RCB block of at most group_size cells). XCD x still takes the
-// tiles t = x (mod 8), but its items run group by group, and within a group tile by tile over the
-// group's cells: each XCD then works on one velocity tile of one compact group at a time, whose rows
-// and neighbour halo fit its L2, where the per-cell order above interleaves a cell's tiles and
-// reaches its farther neighbours (an RCM front, ~900 cells on the 40k tube) only after their rows
-// have left the L2.
-template <class F>
-__device__ __forceinline__ bool work_tile(const View<F>& v, unsigned L, int tiles, int& cell, int& tile) {
-  if (v.group_cells == nullptr) return xcd_tile(L, v.n_cells, tiles, cell, tile);
-  const unsigned x = L & 7u, k = L >> 3;
-  const int ntx = (tiles - static_cast<int>(x) + 7) / 8;
-  if (ntx <= 0) return false;
-  const unsigned gs = static_cast<unsigned>(v.group_size), per_group = gs * static_cast<unsigned>(ntx);
-  const unsigned g = k / per_group;
-  if (g >= static_cast<unsigned>(v.n_groups)) return false;
-  const unsigned r = k - g * per_group, tl = r / gs;
-  const int c = v.group_cells[g * gs + (r - tl * gs)];
-  if (c < 0) return false;
-  cell = c;
-  tile = static_cast<int>(x + 8u * tl);
-  return true;
-}
-
 template <class F, int Mode = 0>  // Mode: limiter branches (0 off, 1 record, 2 follow the record)
 __global__ __launch_bounds__(kBlock) void k_gradient(View<F> v, Chunk<F> ch, int tiles, unsigned L0) {
   int c, tile;
-  if (!work_tile(v, L0 + blockIdx.x, tiles, c, tile)) return;
+  if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   if (t >= ch.kc) return;
   const int p = ch.k0 + t;
@@ -774,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(View<F> v, Chunk<F> ch, int
                                                      double* __restrict__ partial) {
   __shared__ double lds[kWaves];
   int c, tile;
-  if (!work_tile(v, L0 + blockIdx.x, tiles, c, tile)) return;
+  if (!xcd_tile(L0 + blockIdx.x, v.n_cells, tiles, c, tile)) return;
   const int t = tile * kBlock + threadIdx.x;
   double r2[1] = {0.0};
   if (t < ch.kc) {
@@ -1604,7 +1580,6 @@ class HipBackend final : public Backend {
     bsums_.alloc(static_cast<std::size_t>(nb_) * dev::kBoundarySums + 1);
     bsum_chunks_ = std::max(1, std::min(16, (nv_ + 2047) / 2048));
     bsum_part_.alloc(static_cast<std::size_t>(nb_) * bsum_chunks_ * 2 * dev::kSideSums + 1);
-    build_cell_groups(s);
     // chunk width: requested (rounded up to the orbit size), or as wide as 6 workspace arrays fit
     const int orbit = s.orbit_size;
     long long k = s.velocity_chunk > 0 ? s.velocity_chunk : nv_;
@@ -1913,7 +1888,7 @@ class HipBackend final : public Backend {
   static constexpr long long kMaxWorkItems = 0xFFFFFFFFLL;
   long long work_items(long long k) const {
     const long long t = (k + dev::kBlock - 1) / dev::kBlock;
-    return 8LL * ((t + 7) / 8) * (group_size_ > 0 ? static_cast<long long>(n_groups_) * group_size_ : nc_);
+    return 8LL * ((t + 7) / 8) * nc_;
   }
 
   // Workgroups of one launch over `work` items: at most grid_cap_ (a multiple of 8, so a kernel's
@@ -2330,85 +2305,6 @@ class HipBackend final : public Backend {
  private:
   // Dependency levels of every (direction, block) for the forward and the backward sweep, and the
   // storage positions of each direction's velocities (see k_sweep_forward).
-  // Compact cell groups of the gradient / residual launches (work_tile): each RCB block is split by
-  // recursive bisection of its face graph into ceil(n / target) parts of at most target cells
-  // (target 512: a group's rows of one 256-velocity tile and its neighbour halo, four arrays of FP64,
-  // stay within an XCD's 4 MB L2), each part cut off along a breadth-first order from a peripheral
-  // cell so that it is compact. GSK_CELL_GROUP overrides the target (0: the per-cell mapping).
-  // Only the order of the work items changes: every (cell, velocity) is computed as before.
-  void build_cell_groups(const gsk_setup_view& s) {
-    int target = 512;
-    if (const char* env = std::getenv("GSK_CELL_GROUP")) target = std::max(0, std::atoi(env));
-    group_size_ = 0;
-    n_groups_ = 0;
-    if (target <= 0) return;
-    std::vector<int> stamp(static_cast<std::size_t>(nc_), -1), level_order;
-    int epoch = 0;
-    std::vector<std::vector<int>> groups;
-    // breadth-first order of `set` over internal faces between its cells, from `start`; cells the
-    // search does not reach follow, each component searched from its first cell in `set`
-    auto bfs = [&](const std::vector<int>& set, int start) {
-      ++epoch;
-      for (const int c : set) stamp[static_cast<std::size_t>(c)] = epoch;
-      std::vector<int> out;
-      out.reserve(set.size());
-      auto grow = [&](int c0) {
-        std::size_t head = out.size();
-        out.push_back(c0);
-        stamp[static_cast<std::size_t>(c0)] = -epoch;
-        while (head < out.size()) {
-          const int c = out[head++];
-          for (int k = s.csr_offsets[c]; k < s.csr_offsets[c + 1]; ++k) {
-            const int o = s.csr[k].other;
-            if (o < 0 || stamp[static_cast<std::size_t>(o)] != epoch) continue;
-            stamp[static_cast<std::size_t>(o)] = -epoch;
-            out.push_back(o);
-          }
-        }
-      };
-      grow(start);
-      for (const int c : set)
-        if (stamp[static_cast<std::size_t>(c)] == epoch) grow(c);
-      return out;
-    };
-    std::vector<std::pair<std::vector<int>, int>> work;
-    for (int b = 0; b < n_blocks_; ++b) {
-      std::vector<int> cells;
-      for (int c = s.block_offsets[b]; c < s.block_offsets[b + 1]; ++c) cells.push_back(c);
-      if (!cells.empty()) work.emplace_back(std::move(cells), static_cast<int>((s.block_offsets[b + 1] - s.block_offsets[b] + target - 1) / target));
-    }
-    while (!work.empty()) {
-      auto [set, parts] = std::move(work.back());
-      work.pop_back();
-      if (parts <= 1 || set.size() <= 1) {
-        std::sort(set.begin(), set.end());
-        groups.push_back(std::move(set));
-        continue;
-      }
-      const std::vector<int> first = bfs(set, set.front());
-      const std::vector<int> order = bfs(set, first.back());
-      const int p1 = parts / 2;
-      const std::size_t n1 = (set.size() * static_cast<std::size_t>(p1) + static_cast<std::size_t>(parts) / 2) /
-                             static_cast<std::size_t>(parts);
-      work.emplace_back(std::vector<int>(order.begin() + static_cast<std::ptrdiff_t>(n1), order.end()), parts - p1);
-      work.emplace_back(std::vector<int>(order.begin(), order.begin() + static_cast<std::ptrdiff_t>(n1)), p1);
-    }
-    std::sort(groups.begin(), groups.end(), [](const std::vector<int>& a, const std::vector<int>& b) { return a.front() < b.front(); });
-    for (const auto& g : groups) group_size_ = std::max(group_size_, static_cast<int>(g.size()));
-    n_groups_ = static_cast<int>(groups.size());
-    std::vector<int> table(static_cast<std::size_t>(n_groups_) * static_cast<std::size_t>(group_size_), -1);
-    std::vector<char> seen(static_cast<std::size_t>(nc_), 0);
-    for (std::size_t g = 0; g < groups.size(); ++g)
-      for (std::size_t i = 0; i < groups[g].size(); ++i) {
-        const int c = groups[g][i];
-        GSK_CHECK(c >= 0 && c < nc_ && !seen[static_cast<std::size_t>(c)], "cell groups must cover each cell once");
-        seen[static_cast<std::size_t>(c)] = 1;
-        table[g * static_cast<std::size_t>(group_size_) + i] = c;
-      }
-    for (int c = 0; c < nc_; ++c) GSK_CHECK(seen[static_cast<std::size_t>(c)], "cell groups must cover every cell");
-    group_cells_.upload(table.data(), table.size(), stream_);
-  }
-
   void build_levels(const gsk_setup_view& s) {
     n_dirs_eff_ = std::max(1, s.n_sweep_dirs);
     GSK_CHECK(n_dirs_eff_ <= dev::kMaxDirs, "too many sweep directions");
@@ -2659,9 +2555,6 @@ class HipBackend final : public Backend {
     v.cold_bgk = cold_bgk_;
     v.lim_mode = lim_mode_;
     v.lim_branch = branch_.p;
-    v.group_cells = group_size_ > 0 ? group_cells_.p : nullptr;
-    v.n_groups = n_groups_;
-    v.group_size = group_size_;
     return v;
   }
 
@@ -2678,8 +2571,6 @@ class HipBackend final : public Backend {
   DBuf<int> adj_mate_, adj_bslot_, adj_cells_;
   std::vector<int> adj_mate_host_, adj_bslot_host_, host_bface_owner_;
   int n_adj_cells_ = 0;
-  DBuf<int> group_cells_;  // build_cell_groups
-  int n_groups_ = 0, group_size_ = 0;
   int lim_mode_ = 0, max_faces_ = 0;  // limiter branches (set_limiter_branches); most faces of a cell
   DBuf<unsigned> branch_;
   dev::DeqGrid deq_{};

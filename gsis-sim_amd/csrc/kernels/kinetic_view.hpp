@@ -92,9 +92,6 @@ struct View {
   // branch every limited gradient takes; 2 evaluates every limited gradient on its recorded branch
   int lim_mode;
   unsigned* lim_branch;  // [cell][velocity]
-  // compact cell groups of the gradient / residual work mapping (null: cells in order)
-  const int* group_cells;
-  int n_groups, group_size;
 };
 
 // velocity chunk workspace [n_cells][stride], position t = p - k0
