@@ -673,7 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_param_jacobian(View<double> v, c
         Sk[k] += sgn == 0 ? h : -h;
         finalize_moments(Sk, v.weight_d, u0x, u0y, u0z, m);
         int r = 0;
-        const CellP<double> cp = cell_params_block(v, m, volume[c], vdt[c], mu0, omega, Pr, r, lds);
+        const CellP<double> cp = cell_params_wave(v, m, volume[c], vdt[c], mu0, omega, Pr, r);
         params_to_vector(cp, P);
         for (int i = 0; i < kAdjParams; ++i) Pd[i] = sgn == 0 ? P[i] : (Pd[i] - P[i]) / (2.0 * h);
         for (int i = 0; i < kMoments; ++i) md[i] = sgn == 0 ? m[i] : (md[i] - m[i]) / (2.0 * h);

@@ -195,72 +195,73 @@ __device__ int block_discrete_maxwell(int nv, const F* __restrict__ xs, const F*
   return -1;
 }
 
-// The same discrete Maxwellian with one wave per cell: the wave's lanes share the velocity sums
-// (wave_sum leaves the total in every lane), and every lane takes the Newton step itself, so no lane
-// waits on another and the workgroup's other waves (other cells) fill the latency of the step. Same
-// iteration, step control, accept rule and result as block_discrete_maxwell up to the order of the
-// sums.
-template <class F>
-__device__ int wave_discrete_maxwell(int nv, const F* __restrict__ xs, const F* __restrict__ ys,
-                                     const F* __restrict__ zs, double weight, const F* __restrict__ wq,
-                                     const DeqGrid& grid, double tolerance, double rho, double ux, double uy, double uz,
-                                     double T, const Target<F>* poly, Maxw<F>& out, double (&gamma)[5]) {
+// The discrete Maxwellian of one cell by one wave, with the Newton sums in product form
+// (deq_product_newton_sums): the lanes take the nodes of the three axes, form the one-axis sums
+// mu[a][k] = sum_i E_a(i) c_a(i)^k, E_a(i) = exp(-B (x_i - u'_a)^2) times the node's weight factor,
+// and share them through wave_sum (which leaves the totals in every lane), so every lane takes the
+// Newton step itself and no lane waits on another. 3 (n0 + n1 + n2) exponentials per Newton step
+// instead of n0 n1 n2. Same iteration, trial parameters (deq_params<F>, as the f precision stores
+// them), step control, accept rule and result as block_discrete_maxwell, whose point sums these
+// equal up to rounding; the one-axis terms are FP64 in both forms.
+__device__ __forceinline__ void deq_axis_sums(const AxisGrid& g, double B, const double (&up)[3], const double (&u)[3],
+                                              double (&mu)[3][kAxisPowers]) {
   const int lane = static_cast<int>(threadIdx.x & 63);
+  double part[3 * kAxisPowers];
+#pragma unroll
+  for (int k = 0; k < 3 * kAxisPowers; ++k) part[k] = 0.0;
+  int off = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll 1
+    for (int i = lane; i < g.n[a]; i += 64) {
+      double t[kAxisPowers];
+      deq_axis_terms(g.node[off + i], up[a], u[a], B, g.w != nullptr ? g.w[off + i] : 1.0, t);
+#pragma unroll
+      for (int k = 0; k < kAxisPowers; ++k) part[a * kAxisPowers + k] += t[k];
+    }
+    off += g.n[a];
+  }
+  wave_sum<3 * kAxisPowers>(part);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < kAxisPowers; ++k) mu[a][k] = part[a * kAxisPowers + k];
+}
+
+template <class F>
+__device__ int wave_discrete_maxwell(const AxisGrid& g, double weight, const DeqGrid& grid, double tolerance,
+                                     double rho, double ux, double uy, double uz, double T, const Target<F>* poly,
+                                     Maxw<F>& out, double (&gamma)[5]) {
   for (int k = 0; k < 5; ++k) gamma[k] = 0.0;
   out = maxw_from_state<F>(rho, ux, uy, uz, T);
   if (deq_resolved(grid, ux, uy, uz, T)) return 0;
   double beta[5];
   deq_initial_beta(rho, T, beta);
-  const F fux = static_cast<F>(ux), fuy = static_cast<F>(uy), fuz = static_cast<F>(uz);
+  // c = xi - u with u as the f precision holds it (the point form's x - fux)
+  const double u[3] = {static_cast<double>(static_cast<F>(ux)), static_cast<double>(static_cast<F>(uy)),
+                       static_cast<double>(static_cast<F>(uz))};
   double rel_prev = 1e300;
   for (int it = 0; it < grid.max_iterations; ++it) {
     const Maxw<F> m = deq_params<F>(beta, ux, uy, uz);
+    const double up[3] = {static_cast<double>(m.ux), static_cast<double>(m.uy), static_cast<double>(m.uz)};
+    double mu[3][kAxisPowers];
+    deq_axis_sums(g, static_cast<double>(m.B), up, u, mu);
+    const double a = static_cast<double>(m.A) * g.wscale;
     double sums[kDeqSums];
-#pragma unroll
-    for (int k = 0; k < kDeqSums; ++k) sums[k] = 0.0;
-#pragma unroll 1
-    for (int p = lane; p < nv; p += 64) {
-      const F x = xs[p], y = ys[p], z = zs[p];
-      F t[kDeqSums];
-      F e = maxw_eval(m, x, y, z);
-      if (wq != nullptr) e *= wq[p];  // stretched grid: weight * wq[p] per velocity
-      deq_terms<F>(e, F(0), x - fux, y - fuy, z - fuz, t);
-#pragma unroll
-      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] += static_cast<double>(t[k]);
-    }
-    {
-      double s15[kDeqNewtonSums];
-#pragma unroll
-      for (int k = 0; k < kDeqNewtonSums; ++k) s15[k] = sums[k];
-      wave_sum<kDeqNewtonSums>(s15);
-#pragma unroll
-      for (int k = 0; k < kDeqNewtonSums; ++k) sums[k] = s15[k];
-    }
+    deq_product_newton_sums(mu, a, sums);
     double b[5] = {beta[0], beta[1], beta[2], beta[3], beta[4]};
     double rel = 0.0;
     const int r = deq_newton_step(sums, weight, rho, T, tolerance, b, rel);
     if (deq_accept(r, rel, rel_prev, it + 1 == grid.max_iterations)) {
       out = m;
       if (poly != nullptr) {
-        double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int p = lane; p < nv; p += 64) {
-          const F x = xs[p], y = ys[p], z = zs[p];
-          const F cx = x - fux, cy = y - fuy, cz = z - fuz;
-          const F pm = shakhov_poly(*poly, cx, cy, cz, cx * cx + cy * cy + cz * cz);
-          F t[kDeqSums];
-          F e = maxw_eval(m, x, y, z);
-          if (wq != nullptr) e *= wq[p];
-          deq_terms<F>(e, pm, cx, cy, cz, t);
-#pragma unroll
-          for (int k = 0; k < 5; ++k) s5[k] += static_cast<double>(t[kDeqNewtonSums + k]);
-        }
-        wave_sum<5>(s5);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) sums[kDeqNewtonSums + k] = s5[k];
-        double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (!deq_shakhov_correction(sums, weight, g))
-          for (int k = 0; k < 5; ++k) g[k] = 0.0;
-        for (int k = 0; k < 5; ++k) gamma[k] = g[k];
+        const double q[3] = {static_cast<double>(poly->qx), static_cast<double>(poly->qy),
+                             static_cast<double>(poly->qz)};
+        deq_product_shakhov_sums(mu, a, static_cast<double>(poly->coef), q, static_cast<double>(poly->theta), sums);
+        double gm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!deq_shakhov_correction(sums, weight, gm))
+          for (int k = 0; k < 5; ++k) gm[k] = 0.0;
+        for (int k = 0; k < 5; ++k) gamma[k] = gm[k];
       }
       return it + 1;
     }
@@ -487,18 +488,22 @@ __global__ void k_scale_moments(double* __restrict__ moments, int n_cells, doubl
   for (int k = 5; k < kMoments; ++k) m[k] *= s;
 }
 
-// Cell parameters of one cell from its 15 moments, all threads of the block cooperating on the
-// discrete-equilibrium sums; r: the Newton's iterations (0 resolved, -1 failed)
+// Cell parameters of one cell from its 15 moments: Shakhov target, collision rate and diagonal
+// (make_cell_params), then, with the discrete equilibrium, the Newton for its Maxwellian and the
+// Shakhov-polynomial correction gamma (k_cell_params_deq, kinetic_cuda_kernels.cu:232-260); a cell
+// whose Newton fails keeps the continuous target, or with deq_fallback turns collisionless. Computed
+// by one wave (wave_discrete_maxwell); every lane ends with the same result. r: the Newton's
+// iterations (0 resolved, -1 failed).
 template <class F>
-__device__ CellP<F> cell_params_block(const View<F>& v, const double* m, double volume, double vdt, double mu0,
-                                      double omega, double Pr, int& r, double* lds) {
+__device__ CellP<F> cell_params_wave(const View<F>& v, const double* m, double volume, double vdt, double mu0,
+                                     double omega, double Pr, int& r) {
   CellP<F> cp = make_cell_params<F>(m, volume, vdt, mu0, omega, Pr, v.cold_bgk);
   r = 0;
   if (v.discrete_eq) {
     Maxw<F> eq;
     double gamma[5];
-    r = block_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
-                                  m[2], m[3], m[4], &cp.s, eq, gamma, lds);
+    r = wave_discrete_maxwell<F>(v.axes, v.weight_d, v.deq, v.deq.tolerance, m[0], m[1], m[2], m[3], m[4], &cp.s, eq,
+                                 gamma);
     if (r > 0) {
       cp.s.eq = eq;
       cp.s.g0 = static_cast<F>(gamma[0]);
@@ -514,36 +519,18 @@ __device__ CellP<F> cell_params_block(const View<F>& v, const double* m, double 
   return cp;
 }
 
-// Shakhov target, collision rate and diagonal per cell; discrete equilibrium Newton with the
-// Shakhov-polynomial correction gamma when enabled (k_cell_params_deq). One wave per cell
-// (wave_discrete_maxwell): kWaves cells per workgroup, no workgroup barrier.
+// Shakhov target, collision rate and diagonal per cell: one wave per cell (cell_params_wave),
+// kWaves cells per workgroup, no workgroup barrier.
 template <class F>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_cell_params(View<F> v, const double* __restrict__ moments,
+__global__ __launch_bounds__(kBlock) void k_cell_params(View<F> v, const double* __restrict__ moments,
                                                         const double* __restrict__ volume, const double* __restrict__ vdt,
                                                         double mu0, double omega, double Pr, CellP<F>* __restrict__ cells,
                                                         int* __restrict__ flags) {
   const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
   for (int c = blockIdx.x * kWaves + wave; c < v.n_cells; c += gridDim.x * kWaves) {
-    const double* m = moments + static_cast<i64>(c) * kMoments;
-    CellP<F> cp = make_cell_params<F>(m, volume[c], vdt[c], mu0, omega, Pr, v.cold_bgk);
     int r = 0;
-    if (v.discrete_eq) {
-      Maxw<F> eq;
-      double gamma[5];
-      r = wave_discrete_maxwell<F>(v.nv, v.xi_x, v.xi_y, v.xi_z, v.weight_d, v.wq, v.deq, v.deq.tolerance, m[0], m[1],
-                                   m[2], m[3], m[4], &cp.s, eq, gamma);
-      if (r > 0) {
-        cp.s.eq = eq;
-        cp.s.g0 = static_cast<F>(gamma[0]);
-        cp.s.gx = static_cast<F>(gamma[1]);
-        cp.s.gy = static_cast<F>(gamma[2]);
-        cp.s.gz = static_cast<F>(gamma[3]);
-        cp.s.g4 = static_cast<F>(gamma[4]);
-      } else if (r < 0 && v.deq_fallback) {
-        cp.coll = F(0);
-        cp.diag = static_cast<F>(vdt[c]);
-      }
-    }
+    const CellP<F> cp = cell_params_wave(v, moments + static_cast<i64>(c) * kMoments, volume[c], vdt[c], mu0, omega,
+                                         Pr, r);
     if (lane == 0) {
       cells[c] = cp;
       flags[c] = r;
@@ -1542,6 +1529,7 @@ class HipBackend final : public Backend {
       HIPCHK(hipMemsetAsync(specular_first_.p, 0, specular_first_.bytes(), stream_));
       grid_to_storage_.upload(s.grid_to_storage, static_cast<std::size_t>(nv_), stream_);
     }
+    upload_axes(s);
     for (int a = 0; a < 3; ++a) {
       grid_n_[a] = s.grid_n[a];
       deq_.lo[a] = s.deq_lo[a];
@@ -2514,8 +2502,41 @@ class HipBackend final : public Backend {
     return e;
   }
 
+  // The velocity grid axis by axis (dev::AxisGrid) from the storage-order points: node i of an axis is
+  // the point with index i on that axis and 0 on the others (as the f precision stores it); on a
+  // stretched grid the weight factor of that node is the point's relative weight, and since the
+  // relative weight of (i, j, k) is wx(i) wy(j) wz(k) / (wx(0) wy(0) wz(0))^2 times those of the
+  // three points, the product of the three factors is scaled by 1 / w(0, 0, 0)^2.
+  void upload_axes(const gsk_setup_view& s) {
+    const int n0 = s.grid_n[0], n1 = s.grid_n[1], n2 = s.grid_n[2];
+    GSK_CHECK(s.grid_to_storage != nullptr && static_cast<long long>(n0) * n1 * n2 == nv_,
+              "velocity grid shape differs from the velocity count");
+    auto point = [&](int i, int j, int k) { return s.grid_to_storage[i + n0 * (j + n1 * k)]; };
+    std::vector<double> node, w;
+    for (int i = 0; i < n0; ++i) node.push_back(static_cast<double>(static_cast<F>(s.xi_x[point(i, 0, 0)])));
+    for (int j = 0; j < n1; ++j) node.push_back(static_cast<double>(static_cast<F>(s.xi_y[point(0, j, 0)])));
+    for (int k = 0; k < n2; ++k) node.push_back(static_cast<double>(static_cast<F>(s.xi_z[point(0, 0, k)])));
+    axes_.n[0] = n0;
+    axes_.n[1] = n1;
+    axes_.n[2] = n2;
+    axes_.wscale = 1.0;
+    axis_node_.upload(node.data(), node.size(), stream_);
+    if (s.weight_rel != nullptr) {
+      for (int i = 0; i < n0; ++i) w.push_back(s.weight_rel[point(i, 0, 0)]);
+      for (int j = 0; j < n1; ++j) w.push_back(s.weight_rel[point(0, j, 0)]);
+      for (int k = 0; k < n2; ++k) w.push_back(s.weight_rel[point(0, 0, k)]);
+      const double w0 = s.weight_rel[point(0, 0, 0)];
+      GSK_CHECK(w0 > 0.0, "velocity grid weight is not positive");
+      axes_.wscale = 1.0 / (w0 * w0);
+      axis_w_.upload(w.data(), w.size(), stream_);
+    }
+    axes_.node = axis_node_.p;
+    axes_.w = axis_w_.p;
+  }
+
   dev::View<F> view() {
     dev::View<F> v{};
+    v.axes = axes_;
     v.n_cells = nc_;
     v.nv = nv_;
     v.n_bfaces = nb_;
@@ -2578,6 +2599,8 @@ class HipBackend final : public Backend {
   DBuf<double> xd_, yd_, zd_;
   DBuf<F> wq_;                   // stretched grid only (relative weights; empty: uniform)
   DBuf<double> wqd_, nodes_;
+  dev::AxisGrid axes_{};
+  DBuf<double> axis_node_, axis_w_;  // the grid axis by axis (upload_axes)
   DBuf<int> csr_off_;
   DBuf<dev::CsrE<F>> csr_;
   DBuf<dev::EllRow<F>> ell_;

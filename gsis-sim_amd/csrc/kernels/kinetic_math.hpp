@@ -314,6 +314,79 @@ GSK_HD void deq_terms(F e, F pm, F cx, F cy, F cz, F (&t)[kDeqSums]) {
   t[19] = ep * h;
 }
 
+// The same sums on a tensor-product velocity grid in product form. There the trial Maxwellian
+// A exp(-B |xi - u'|^2), times the point's relative weight, is A times one factor E_a per axis, and
+// every term of deq_terms is e times a monomial cx^i cy^j cz^k in c = xi - u (degree <= 4 for the
+// 15 Newton sums; the Shakhov polynomial, cubic in c, times 1, c or |c|^2 / 2 for the last five:
+// degree <= 5). A sum over all n0 n1 n2 points of such a term is therefore
+//   A * mu[0][i] * mu[1][j] * mu[2][k],   mu[a][k] = sum over the nodes of axis a of E_a c_a^k,
+// so one Newton step needs 3 (n0 + n1 + n2) one-axis terms instead of n0 n1 n2 point terms, and
+// gives the same sums up to rounding. a = A times the weight normalisation.
+inline constexpr int kAxisPowers = 6;  // c^0 .. c^5
+
+// one node's terms of the one-axis sums: E c^k for k = 0..5, E = exp(-B (x - up)^2) w, c = x - u
+// (up: the trial Maxwellian's velocity on this axis, u: the cell's, w: the node's weight factor)
+GSK_HD void deq_axis_terms(double x, double up, double u, double B, double w, double (&t)[kAxisPowers]) {
+  const double d = x - up;
+  double e = ::exp(-B * (d * d)) * w;
+  const double c = x - u;
+  for (int k = 0; k < kAxisPowers; ++k) {
+    t[k] = e;
+    e *= c;
+  }
+}
+
+GSK_HD double deq_monomial(const double (&mu)[3][kAxisPowers], double a, int i, int j, int k) {
+  return a * (mu[0][i] * mu[1][j] * mu[2][k]);
+}
+
+// s[0..15) in deq_terms' order: e {1, cx, cy, cz, h, cx cx, cx cy, cx cz, cx h, cy cy, cy cz, cy h,
+// cz cz, cz h, h h}, h = |c|^2 / 2
+GSK_HD void deq_product_newton_sums(const double (&mu)[3][kAxisPowers], double a, double* s) {
+  s[0] = deq_monomial(mu, a, 0, 0, 0);
+  s[1] = deq_monomial(mu, a, 1, 0, 0);
+  s[2] = deq_monomial(mu, a, 0, 1, 0);
+  s[3] = deq_monomial(mu, a, 0, 0, 1);
+  s[4] = 0.5 * (deq_monomial(mu, a, 2, 0, 0) + deq_monomial(mu, a, 0, 2, 0) + deq_monomial(mu, a, 0, 0, 2));
+  s[5] = deq_monomial(mu, a, 2, 0, 0);
+  s[6] = deq_monomial(mu, a, 1, 1, 0);
+  s[7] = deq_monomial(mu, a, 1, 0, 1);
+  s[8] = 0.5 * (deq_monomial(mu, a, 3, 0, 0) + deq_monomial(mu, a, 1, 2, 0) + deq_monomial(mu, a, 1, 0, 2));
+  s[9] = deq_monomial(mu, a, 0, 2, 0);
+  s[10] = deq_monomial(mu, a, 0, 1, 1);
+  s[11] = 0.5 * (deq_monomial(mu, a, 2, 1, 0) + deq_monomial(mu, a, 0, 3, 0) + deq_monomial(mu, a, 0, 1, 2));
+  s[12] = deq_monomial(mu, a, 0, 0, 2);
+  s[13] = 0.5 * (deq_monomial(mu, a, 2, 0, 1) + deq_monomial(mu, a, 0, 2, 1) + deq_monomial(mu, a, 0, 0, 3));
+  s[14] = 0.25 * (deq_monomial(mu, a, 4, 0, 0) + deq_monomial(mu, a, 0, 4, 0) + deq_monomial(mu, a, 0, 0, 4) +
+                  2.0 * (deq_monomial(mu, a, 2, 2, 0) + deq_monomial(mu, a, 2, 0, 2) + deq_monomial(mu, a, 0, 2, 2)));
+}
+
+// sum of e pm c^b, pm = coef (c . q)(|c|^2 / theta - 5) = coef sum_d q_d c_d (sum_g c_g^2 / theta - 5)
+GSK_HD double deq_product_shakhov_term(const double (&mu)[3][kAxisPowers], double a, double coef, const double (&q)[3],
+                                       double theta, int bx, int by, int bz) {
+  double total = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const int e[3] = {bx + (d == 0), by + (d == 1), bz + (d == 2)};
+    double cube = 0.0;
+    for (int g = 0; g < 3; ++g)
+      cube += deq_monomial(mu, a, e[0] + 2 * (g == 0), e[1] + 2 * (g == 1), e[2] + 2 * (g == 2));
+    total += q[d] * (cube / theta - 5.0 * deq_monomial(mu, a, e[0], e[1], e[2]));
+  }
+  return coef * total;
+}
+
+// s[15..20): e pm {1, cx, cy, cz, h}
+GSK_HD void deq_product_shakhov_sums(const double (&mu)[3][kAxisPowers], double a, double coef, const double (&q)[3],
+                                     double theta, double* s) {
+  s[15] = deq_product_shakhov_term(mu, a, coef, q, theta, 0, 0, 0);
+  s[16] = deq_product_shakhov_term(mu, a, coef, q, theta, 1, 0, 0);
+  s[17] = deq_product_shakhov_term(mu, a, coef, q, theta, 0, 1, 0);
+  s[18] = deq_product_shakhov_term(mu, a, coef, q, theta, 0, 0, 1);
+  s[19] = 0.5 * (deq_product_shakhov_term(mu, a, coef, q, theta, 2, 0, 0) +
+                 deq_product_shakhov_term(mu, a, coef, q, theta, 0, 2, 0) +
+                 deq_product_shakhov_term(mu, a, coef, q, theta, 0, 0, 2));
+}
+
 GSK_HD void deq_jacobian(const double* s, double w, double (&J)[5][5]) {
   int k = 0;
   for (int a = 0; a < 5; ++a)

@@ -50,9 +50,21 @@ struct Inflow {
   F scale;
 };
 
+// The tensor velocity grid axis by axis, for sums in product form (deq_product_newton_sums): the
+// node coordinates of axes x, y, z back to back (the values the f precision stores) and, on a
+// stretched grid, one weight factor per node such that the relative weight of point (i, j, k) is
+// w[i] * w[n0 + j] * w[n0 + n1 + k] * wscale (w null and wscale 1 on the uniform grid).
+struct AxisGrid {
+  const double* node;
+  const double* w;
+  double wscale;
+  int n[3];
+};
+
 // All device pointers and scalars the kernels read; passed by value as a kernel argument.
 template <class F>
 struct View {
+  AxisGrid axes;
   int n_cells, nv, n_bfaces;
   const F* xi_x;
   const F* xi_y;
