@@ -122,7 +122,8 @@ EXPORTS = [
     "gsk_solver_residual_per_cell", "gsk_solver_section_mass_flow", "gsk_solver_section_cuts", "gsk_solver_backend", "gsk_solver_element_size", "gsk_solver_distribution_device",
     "gsk_solver_set_distribution_device", "gsk_solver_residual_device", "gsk_solver_precondition_device",
     "gsk_solver_mu0", "gsk_solver_set_mu0", "gsk_solver_set_accommodation", "gsk_solver_set_newton_tolerances",
-    "gsk_solver_set_boundary_iterations", "gsk_solver_freeze_limiters", "gsk_solver_adjoint_prepare",
+    "gsk_solver_set_boundary_iterations", "gsk_solver_set_device_prepare", "gsk_solver_device_prepare",
+    "gsk_solver_freeze_limiters", "gsk_solver_adjoint_prepare",
     "gsk_solver_adjoint_apply_device",
 ]
 
@@ -485,6 +486,17 @@ class Solver:
 
     def set_boundary_iterations(self, n: int) -> None:
         _check(lib().gsk_solver_set_boundary_iterations(self._h, C.c_int64(n)))
+
+    def set_device_prepare(self, on: bool) -> None:
+        """Run each iteration's prepare on the backend's device (on, the default where it offers one)
+        or on the host."""
+        _check(lib().gsk_solver_set_device_prepare(self._h, C.c_int32(1 if on else 0)))
+
+    def prepared_on_device(self) -> bool:
+        """Whether the last prepare ran on the backend's device."""
+        out = C.c_int32(0)
+        _check(lib().gsk_solver_device_prepare(self._h, C.byref(out)))
+        return bool(out.value)
 
     def freeze_limiters(self, on: bool) -> None:
         """Record every limiter branch at the next residual_device and follow it afterwards (on), or

@@ -744,6 +744,21 @@ int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n) {
   });
 }
 
+int gsk_solver_set_device_prepare(gsk_solver* sv, int32_t on) {
+  return guarded([&] {
+    need(sv, "solver");
+    sv->sv->set_device_prepare(on != 0);
+  });
+}
+
+int gsk_solver_device_prepare(gsk_solver* sv, int32_t* on_device) {
+  return guarded([&] {
+    need(sv, "solver");
+    need(on_device, "on_device");
+    *on_device = sv->sv->prepared_on_device() ? 1 : 0;
+  });
+}
+
 gsk_backend* gsk_solver_backend(gsk_solver* sv) {
   if (!sv) return nullptr;
   sv->handle.b = &sv->sv->backend();

@@ -71,6 +71,7 @@ void KineticSolver::adjoint_prepare() {
   for (const PatchCondition& pc : bnd_.patches())
     GSK_CHECK(pc.type != BoundaryType::MassFlowInlet, "adjoint operators do not cover mass-flow inlets");
   prepare();
+  GSK_CHECK(!prepared_on_device_, "the adjoint tables need the host's boundary state of the last prepare");
   auto A = std::make_shared<AdjointTables>();
   const int nb = s_.n_bfaces(), nc = s_.n_cells();
   const auto unb = static_cast<std::size_t>(nb);

@@ -6,10 +6,62 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../../include/gsis_kinetic.h"
 
 namespace gsk {
+
+// The prepare of one CIS iteration on the device (Backend::control_prepare): the tables it needs,
+// built once by the solver from the setup (FP64), and what one call takes and returns.
+struct ControlTables {
+  // per boundary face: owner cell, patch, area / L_ref^2, unit normal (3)
+  std::vector<int> face_owner, face_patch;
+  std::vector<double> face_area, face_normal;
+  // pressure faces and their owners' reconstruction stencils: face index, owner, first slot and
+  // slot count, K^3 V of the owner, face centre - owner centre (3)
+  std::vector<int> press_face, press_owner, press_slot0, press_nslots;
+  std::vector<double> press_eps2, press_to_face;
+  // stencil slots in the owner's CSR order: neighbour cell (or -1) and boundary face (or -1),
+  // whether the face is periodic, LSQ coefficients (3), face centre - owner centre (3)
+  std::vector<int> slot_cell, slot_bface, slot_periodic;
+  std::vector<double> slot_lsq, slot_offset;
+  std::vector<double> volume;     // per cell
+  std::vector<double> wall_unit;  // 10 per boundary face: unit-density half-range sums of Maxwell walls
+  double gamma = 5.0 / 3.0, u_floor = 0.0, relaxation = 1.0;
+  bool limiter = true, cell_reconstruction = false, closed_domain = false;
+};
+
+struct ControlPatch {
+  int kind = 0, inlet = 0, zero_tangential = 0, first = 0, n = 0;
+  double p = 0.0, T = 1.0, accommodation = 1.0;
+};
+
+struct ControlStep {
+  // in: the host's boundary state (inflow: one per boundary face, face_state: 5 per face) and the
+  // iteration's settings; out: the same state after the boundary rounds
+  const ControlPatch* patches = nullptr;
+  std::size_t n_patches = 0;
+  gsk_inflow_state* inflow = nullptr;
+  double* face_state = nullptr;
+  std::size_t n_faces = 0;
+  bool has_face_state = false, has_prev = false, seed_walls = false;
+  long long rounds = 1;
+  // in/out: the closed-domain target mass (set by the first prepare after initialisation)
+  bool has_mass_target = false;
+  double mass_target = 0.0;
+  // out: fluxes F+ / F- (5 per face), patch mass flows, positivity, mass, closed-domain scale,
+  // epsilon parts and volume means
+  double* flux_out = nullptr;
+  double* flux_in = nullptr;
+  double* patch_flow = nullptr;
+  long long bad = 0;
+  int first_bad = -1;
+  double bad_rho = 0.0, bad_T = 0.0;
+  double mass = 0.0, scale = 1.0, eps[3] = {0.0, 0.0, 0.0}, q_mean[3] = {0.0, 0.0, 0.0};
+  double sigma_mean[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  bool maxwell_negative = false, maxwell_no_incoming = false;
+};
 
 class Backend {
  public:
@@ -80,6 +132,34 @@ class Backend {
     (void)faces;
     (void)n;
     (void)out;
+  }
+
+  // The prepare of a CIS iteration kept on the device (the reference's KineticSolver::prepare,
+  // kinetic_solver.cpp:365-452, without its per-step host round trips): control_setup hands over the
+  // tables once and returns false when the backend does not provide it. control_prepare then takes
+  // the moments, the positivity check, mass and closed-domain rescaling, epsilon against the moments
+  // kept by keep_previous_moments, the volume means, the cell parameters and `rounds` boundary rounds
+  // (pressure-face states, their discrete correction, boundary sums, fluxes, face states, wall
+  // densities, patch flows), and brings back only the boundary state and the step's scalars. The
+  // moments stay on the device until download_moments; previous_moments brings back the kept ones,
+  // set_previous_moments hands over moments the host computed.
+  virtual bool control_setup(const ControlTables& t) {
+    (void)t;
+    return false;
+  }
+  virtual void control_prepare(ControlStep& step) { (void)step; }
+  virtual void download_moments(double* out, std::size_t n) {
+    (void)out;
+    (void)n;
+  }
+  virtual void keep_previous_moments() {}
+  virtual void previous_moments(double* out, std::size_t n) {
+    (void)out;
+    (void)n;
+  }
+  virtual void set_previous_moments(const double* m, std::size_t n) {
+    (void)m;
+    (void)n;
   }
 
   // Device-resident operators for sensitivity solves (no counterpart in the reference backend).

@@ -51,6 +51,8 @@ class BoundaryModel {
   void set_branches(Branches b) { branches_ = b; }
   void forget_face_states() { has_face_state_ = false; }
   bool has_face_states() const { return has_face_state_; }
+  // the face states were set elsewhere (the device prepare, which runs the same rounds)
+  void mark_face_states() { has_face_state_ = true; }
 
   std::vector<gsk_inflow_state> inflow;
   std::vector<std::array<double, 5>> flux_out, flux_in, face_state;  // F+, F-, (rho, u, T) per face

@@ -31,6 +31,149 @@ KineticSolver::KineticSolver(const KineticSetup& setup, std::unique_ptr<Backend>
   bsums_.assign(static_cast<std::size_t>(s_.n_bfaces()) * 20, 0.0);
   residual_cell_.assign(nc, 0.0);
   bnd_.reset(inflow_tol_);
+  build_device_control();
+}
+
+// The tables of the device prepare: per boundary face its owner, patch, area and normal; per pressure
+// face the stencil of its owner's reconstruction (BoundaryModel::reconstruct_face_state: the owner's
+// CSR faces in order, with their LSQ coefficients and face-centre offsets); volumes; the Maxwell
+// walls' unit-density sums.
+void KineticSolver::build_device_control() {
+  ControlTables t;
+  const Index n_int = s_.mesh.n_internal_faces(), nb = s_.n_bfaces();
+  const double L = s_.length_scale;
+  for (Index b = 0; b < nb; ++b) {
+    const auto f = static_cast<std::size_t>(b + n_int);
+    t.face_owner.push_back(static_cast<int>(s_.mesh.owner[f]));
+    t.face_patch.push_back(static_cast<int>(s_.bface_patch[static_cast<std::size_t>(b)]));
+    t.face_area.push_back(s_.geometry.face_area[f] / (L * L));
+    const Vec3& n = s_.geometry.face_normal[f];
+    t.face_normal.insert(t.face_normal.end(), {n.x, n.y, n.z});
+  }
+  for (const PatchCondition& pc : bnd_.patches()) {
+    if (pc.kind != kBcPressure) continue;
+    for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) {
+      const auto face = static_cast<std::size_t>(b + n_int);
+      const Index owner = s_.mesh.owner[face];
+      const Vec3& x_own = s_.geometry.cell_center[static_cast<std::size_t>(owner)];
+      const Offset k0 = s_.csr.begin(owner), k1 = s_.csr.end(owner);
+      t.press_face.push_back(static_cast<int>(b));
+      t.press_owner.push_back(static_cast<int>(owner));
+      t.press_slot0.push_back(static_cast<int>(t.slot_cell.size()));
+      t.press_nslots.push_back(static_cast<int>(k1 - k0));
+      const double K = s_.params.limiter_k;
+      t.press_eps2.push_back(K * K * K * s_.volume[static_cast<std::size_t>(owner)]);
+      const Vec3 to_face = s_.geometry.face_center[face] - x_own;
+      t.press_to_face.insert(t.press_to_face.end(), {to_face.x, to_face.y, to_face.z});
+      for (Offset k = k0; k < k1; ++k) {
+        const auto e = static_cast<std::size_t>(k);
+        const Index other = s_.csr.other[e];
+        int bface = -1, periodic = 0;
+        if (other == kInvalid) {
+          bface = static_cast<int>(s_.csr.faces[e] - n_int);
+          periodic = patches_kind(bface) == kBcPeriodic ? 1 : 0;
+        }
+        t.slot_cell.push_back(other == kInvalid ? -1 : static_cast<int>(other));
+        t.slot_bface.push_back(bface);
+        t.slot_periodic.push_back(periodic);
+        const Vec3& l = s_.lsq[e];
+        t.slot_lsq.insert(t.slot_lsq.end(), {l.x, l.y, l.z});
+        const Vec3 off = s_.geometry.face_center[static_cast<std::size_t>(s_.csr.faces[e])] - x_own;
+        t.slot_offset.insert(t.slot_offset.end(), {off.x, off.y, off.z});
+      }
+    }
+  }
+  t.volume = s_.volume;
+  if (!bnd_.wall_unit.empty()) {
+    t.wall_unit.resize(static_cast<std::size_t>(nb) * 10);
+    for (std::size_t b = 0; b < static_cast<std::size_t>(nb); ++b)
+      for (std::size_t k = 0; k < 10; ++k) t.wall_unit[b * 10 + k] = bnd_.wall_unit[b][k];
+  }
+  t.gamma = s_.gamma;
+  t.u_floor = s_.params.convergence_u_floor;
+  t.relaxation = s_.params.wall_relaxation;
+  t.limiter = s_.limiter_mode != 0;
+  t.cell_reconstruction = s_.params.pressure_boundary_reconstruction == "cell";
+  t.closed_domain = s_.closed_domain;
+  control_ = backend_->control_setup(t);
+}
+
+BoundaryKind KineticSolver::patches_kind(int bface) const {
+  return bnd_.patches()[static_cast<std::size_t>(s_.bface_patch[static_cast<std::size_t>(bface)])].kind;
+}
+
+const std::vector<double>& KineticSolver::host_moments() {
+  if (!moments_on_host_) {
+    backend_->download_moments(moments_.data(), moments_.size());
+    moments_on_host_ = true;
+  }
+  return moments_;
+}
+
+// prepare on the device: the same stages, one exchange of the boundary state and the step's scalars
+void KineticSolver::prepare_on_device() {
+  static_assert(sizeof(std::array<double, 5>) == 5 * sizeof(double), "face arrays must be flat");
+  std::vector<ControlPatch> patches;
+  for (const PatchCondition& pc : bnd_.patches()) {
+    ControlPatch c;
+    c.kind = pc.kind;
+    c.inlet = pc.type == BoundaryType::PressureInlet || pc.type == BoundaryType::MassFlowInlet;
+    c.zero_tangential = pc.zero_tangential ? 1 : 0;
+    c.first = static_cast<int>(pc.first_bface);
+    c.n = static_cast<int>(pc.n_faces);
+    c.p = pc.p;
+    c.T = pc.T;
+    c.accommodation = pc.accommodation;
+    patches.push_back(c);
+  }
+  if (has_prev_ && !prev_on_device_) backend_->set_previous_moments(prev_moments_.data(), prev_moments_.size());
+  ControlStep st;
+  st.patches = patches.data();
+  st.n_patches = patches.size();
+  st.inflow = bnd_.inflow.data();
+  st.face_state = bnd_.face_state.empty() ? nullptr : bnd_.face_state[0].data();
+  st.flux_out = bnd_.flux_out.empty() ? nullptr : bnd_.flux_out[0].data();
+  st.flux_in = bnd_.flux_in.empty() ? nullptr : bnd_.flux_in[0].data();
+  st.n_faces = bnd_.inflow.size();
+  st.patch_flow = bnd_.patch_flow.data();
+  st.has_face_state = bnd_.has_face_states();
+  st.seed_walls = !bnd_.has_face_states();
+  st.has_prev = has_prev_;
+  st.rounds = boundary_iterations_;
+  st.has_mass_target = has_mass_target_;
+  st.mass_target = mass_target_;
+  backend_->control_prepare(st);
+  moments_on_host_ = false;
+  prepared_on_device_ = true;
+  if (st.bad > 0)
+    throw Error(strf("iteration %lld: %lld cells have non-positive density or temperature (e.g. original cell %d, rho = "
+                     "%.4g, T = %.4g)",
+                     iteration_, st.bad, s_.cell_new_to_old[static_cast<std::size_t>(st.first_bad)], st.bad_rho,
+                     st.bad_T));
+  info_.total_mass = st.mass;
+  if (s_.closed_domain) {
+    has_mass_target_ = st.has_mass_target;
+    mass_target_ = st.mass_target;
+    info_.mass_scale = st.scale;
+  }
+  if (has_prev_) {
+    info_.eps_rho = st.eps[0];
+    info_.eps_u = st.eps[1];
+    info_.eps_T = st.eps[2];
+    info_.epsilon = std::max({info_.eps_rho, info_.eps_u, info_.eps_T});
+    info_.has_epsilon = 1;
+  }
+  for (int k = 0; k < 3; ++k) info_.q_mean[k] = st.q_mean[k];
+  for (int k = 0; k < 6; ++k) info_.sigma_mean[k] = st.sigma_mean[k];
+  if (s_.discrete_eq) {
+    const gsk_deq_counts dc = backend_->deq_counts();
+    info_.deq_corrected = dc.corrected;
+    info_.deq_failed = dc.failed;
+    info_.deq_max_iterations = dc.max_iterations;
+  }
+  GSK_CHECK(!st.maxwell_no_incoming, "Maxwell wall has no incoming velocity points");
+  GSK_CHECK(!st.maxwell_negative, "Maxwell wall diffuse density became negative; refine or widen the velocity grid");
+  bnd_.mark_face_states();
 }
 
 void KineticSolver::reset_after_initialization() {
@@ -44,6 +187,7 @@ void KineticSolver::reset_after_initialization() {
 // Moments of the current f; every cell must keep a positive, finite density and temperature.
 void KineticSolver::read_moments() {
   backend_->compute_moments(moments_.data(), moments_.size());
+  moments_on_host_ = true;
   Index bad = 0, first = -1;
   for (Index c = 0; c < s_.n_cells(); ++c) {
     const double* m = &moments_[static_cast<std::size_t>(c) * 15];
@@ -84,6 +228,10 @@ void KineticSolver::rescale_closed_domain() {
 // eq. (59): epsilon = max over (rho, u, T) of the volume-weighted relative L2 change since the last
 // iteration; the velocity change is measured against max(|u|, convergence_u_floor).
 void KineticSolver::measure_convergence() {
+  if (prev_on_device_) {
+    backend_->previous_moments(prev_moments_.data(), prev_moments_.size());
+    prev_on_device_ = false;
+  }
   double change[3] = {0, 0, 0}, size[3] = {0, 0, 0}, vol = 0.0;
   for (Index c = 0; c < s_.n_cells(); ++c) {
     const double V = s_.volume[static_cast<std::size_t>(c)];
@@ -156,18 +304,23 @@ const gsk_iteration_info& KineticSolver::prepare() {  // (:365-452)
   info_ = gsk_iteration_info{};
   info_.iteration = iteration_;
   info_.mass_scale = 1.0;
-  read_moments();
-  rescale_closed_domain();
-  if (has_prev_) measure_convergence();
-  iteration_statistics();
-  backend_->update_cell_params();
-  if (s_.discrete_eq) {
-    const gsk_deq_counts dc = backend_->deq_counts();
-    info_.deq_corrected = dc.corrected;
-    info_.deq_failed = dc.failed;
-    info_.deq_max_iterations = dc.max_iterations;
+  if (device_control()) {
+    prepare_on_device();
+  } else {
+    prepared_on_device_ = false;
+    read_moments();
+    rescale_closed_domain();
+    if (has_prev_) measure_convergence();
+    iteration_statistics();
+    backend_->update_cell_params();
+    if (s_.discrete_eq) {
+      const gsk_deq_counts dc = backend_->deq_counts();
+      info_.deq_corrected = dc.corrected;
+      info_.deq_failed = dc.failed;
+      info_.deq_max_iterations = dc.max_iterations;
+    }
+    boundary_rounds();
   }
-  boundary_rounds();
   prepared_version_ = version_;
   info_.seconds = now_seconds() - t0;
   return info_;
@@ -211,7 +364,7 @@ void KineticSolver::update_mass_flow_pressure() {
     double D = 0.0;
     for (Index b = pc.first_bface; b < pc.first_bface + pc.n_faces; ++b) {
       const auto f = static_cast<std::size_t>(b + s_.mesh.n_internal_faces());
-      const double T_cell = moments_[static_cast<std::size_t>(s_.mesh.owner[f]) * 15 + 4];
+      const double T_cell = host_moments()[static_cast<std::size_t>(s_.mesh.owner[f]) * 15 + 4];
       D += s_.geometry.face_area[f] / L2 * (1.0 / std::sqrt(2.0 * kPi * pc.T) + 0.5 / std::sqrt(s_.gamma * T_cell));
     }
     GSK_CHECK(D > 0.0, "mass-flow inlet has zero area");
@@ -251,7 +404,13 @@ const gsk_iteration_info& KineticSolver::sweep(bool update_pressure) {  // (:503
   }
   info_.residual = std::sqrt(sum * s_.velocity.weight / vol);
   info_.has_residual = 1;
-  prev_moments_ = moments_;
+  if (prepared_on_device_) {
+    backend_->keep_previous_moments();
+    prev_on_device_ = true;
+  } else {
+    prev_moments_ = moments_;
+    prev_on_device_ = false;
+  }
   has_prev_ = true;
   ++version_;
   ++iteration_;
@@ -292,9 +451,10 @@ void KineticSolver::moments(double* out, std::size_t n) {
   prepare();
   const auto nc = static_cast<std::size_t>(s_.n_cells());
   GSK_CHECK(n == nc * 15, "moments output length differs from 15 x cells");
+  const std::vector<double>& m = host_moments();
   for (std::size_t c = 0; c < nc; ++c) {
     const auto o = static_cast<std::size_t>(s_.cell_new_to_old[c]);
-    for (std::size_t k = 0; k < 15; ++k) out[o * 15 + k] = moments_[c * 15 + k];
+    for (std::size_t k = 0; k < 15; ++k) out[o * 15 + k] = m[c * 15 + k];
   }
 }
 

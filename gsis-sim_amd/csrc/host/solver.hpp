@@ -75,6 +75,12 @@ class KineticSolver {
   // reconstruction reads the boundary face states of the previous round, so R(f) is a function of f
   // alone once these rounds have converged
   void set_boundary_iterations(long long n);
+  // the prepare on the backend's device (the default where it offers one) or on the host
+  void set_device_prepare(bool on) {
+    use_device_ = on;
+    ++version_;
+  }
+  bool prepared_on_device() const { return prepared_on_device_; }
   // Limiter branches for sensitivity solves: on = the next residual_device records the branch of
   // every limiter (the kinetic limited gradients and the pressure-face reconstruction) at the
   // current f, and later evaluations follow that record, so R is smooth around the recorded state;
@@ -92,6 +98,14 @@ class KineticSolver {
 
  private:
   void reset_after_initialization();
+  // the prepare on the backend's device (Backend::control_prepare) when it offers one and no
+  // limiter branches are being recorded or followed (those stay with the host's boundary model)
+  void build_device_control();
+  bool device_control() const { return control_ && use_device_ && freeze_ == Freeze::Off; }
+  void prepare_on_device();
+  // the moments of the last prepare on the host (the device prepare leaves them on the device)
+  const std::vector<double>& host_moments();
+  BoundaryKind patches_kind(int bface) const;
   // prepare, in stages
   void read_moments();
   void rescale_closed_domain();
@@ -116,6 +130,11 @@ class KineticSolver {
   gsk_iteration_info info_{};
   std::vector<double> moments_, prev_moments_;  // 15 per cell, internal numbering
   bool has_prev_ = false;
+  bool control_ = false;            // the backend can run the prepare on its device
+  bool use_device_ = true;          // and is to
+  bool prepared_on_device_ = false;  // the last prepare ran there
+  bool moments_on_host_ = true;      // moments_ holds the last prepare's moments
+  bool prev_on_device_ = false;      // the previous iteration's moments are the backend's, not prev_moments_
   std::vector<double> bsums_;
   std::vector<double> residual_cell_, inner_residual_;
   double mass_target_ = 0.0;

@@ -294,14 +294,15 @@ __device__ Maxw<F> equilibrium_params(const View<F>& v, double rho, double ux, d
 // {scale, rho, ux, uy, uz, T}: a state that is not resolved by the grid is replaced by the
 // Maxwellian whose discrete moments match it (KineticSolver::correct_inflow, kinetic_solver.cpp:103-114;
 // maxwell_params_to_state, kinetic_kernels.hpp:543-552). Newton tolerance tol (1e-10 as the reference).
+// index: the states to correct (null: the first n).
 __global__ __launch_bounds__(kBlock) void k_inflow_correct(int nv, const double* __restrict__ xd,
                                                            const double* __restrict__ yd, const double* __restrict__ zd,
                                                            double weight, const double* __restrict__ wqd,
                                                            DeqGrid grid, double tol, double* __restrict__ states,
-                                                           int n) {
+                                                           const int* __restrict__ index, int n) {
   __shared__ double lds[kWaves * kDeqSums];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    double* st = states + static_cast<i64>(i) * 6;
+    double* st = states + static_cast<i64>(index != nullptr ? index[i] : i) * 6;
     const double rho = st[1], ux = st[2], uy = st[3], uz = st[4], T = st[5];
     if (!(rho > 0.0) || !(T > 0.0)) continue;
     Maxw<double> m;
@@ -1314,6 +1315,7 @@ __global__ void k_to_double(const F* __restrict__ src, double* __restrict__ dst,
 }  // namespace gsk
 
 #include "kinetic_adjoint.hpp"
+#include "kinetic_control.hpp"
 
 namespace gsk {
 
@@ -1699,6 +1701,12 @@ class HipBackend final : public Backend {
 
   void compute_moments(double* out, std::size_t n) override {
     GSK_CHECK(n == moments_.n, "moments output length differs from 15 x cells");
+    launch_moments();
+    download(out, moments_.p, n);
+  }
+
+  // the 13 shifted sums -> moments_ (and the shift), timed as the moment phase
+  void launch_moments() {
     begin(kMoments);
     if constexpr (std::is_same_v<F, float>) {
       if (moments_mfma_)
@@ -1710,7 +1718,6 @@ class HipBackend final : public Backend {
     }
     HIPCHK(hipGetLastError());
     end();
-    download(out, moments_.p, n);
   }
 
   void scale_distribution(double s) override {
@@ -1812,7 +1819,8 @@ class HipBackend final : public Backend {
     begin(kOther);
     inflow_stage_.upload(reinterpret_cast<const double*>(states), n * 6, stream_);
     dev::k_inflow_correct<<<static_cast<unsigned>(std::min<std::size_t>(n, 65535)), dev::kBlock, 0, stream_>>>(
-        nv_, xd_.p, yd_.p, zd_.p, weight_, wqd_.p, view().deq, inflow_tol_, inflow_stage_.p, static_cast<int>(n));
+        nv_, xd_.p, yd_.p, zd_.p, weight_, wqd_.p, view().deq, inflow_tol_, inflow_stage_.p, nullptr,
+        static_cast<int>(n));
     HIPCHK(hipGetLastError());
     end();
     download(reinterpret_cast<double*>(states), inflow_stage_.p, n * 6);
@@ -2036,6 +2044,184 @@ class HipBackend final : public Backend {
       GSK_CHECK(mode == 1 || lim_mode_ != 0, "no limiter branches have been recorded");
     }
     lim_mode_ = mode;
+  }
+
+  // ---- the prepare of a CIS iteration on the device (kinetic_control.hpp) ----
+  bool control_setup(const ControlTables& t) override {
+    const auto nb = static_cast<std::size_t>(nb_);
+    GSK_CHECK(t.face_owner.size() == nb && t.face_patch.size() == nb && t.face_area.size() == nb &&
+                  t.face_normal.size() == 3 * nb && t.volume.size() == static_cast<std::size_t>(nc_),
+              "control tables do not match the setup");
+    const std::size_t npf = t.press_face.size(), nsl = t.slot_cell.size();
+    GSK_CHECK(t.press_owner.size() == npf && t.press_slot0.size() == npf && t.press_nslots.size() == npf &&
+                  t.press_eps2.size() == npf && t.press_to_face.size() == 3 * npf && t.slot_bface.size() == nsl &&
+                  t.slot_periodic.size() == nsl && t.slot_lsq.size() == 3 * nsl && t.slot_offset.size() == 3 * nsl,
+              "pressure-face tables are inconsistent");
+    std::vector<dev::CtrlFace> faces(nb);
+    for (std::size_t b = 0; b < nb; ++b) {
+      GSK_CHECK(t.face_owner[b] >= 0 && t.face_owner[b] < nc_ && t.face_patch[b] >= 0, "boundary face table out of range");
+      ctl_max_patch_ = std::max(ctl_max_patch_, t.face_patch[b]);
+      faces[b] = dev::CtrlFace{t.face_owner[b], t.face_patch[b], t.face_area[b], t.face_normal[3 * b],
+                               t.face_normal[3 * b + 1], t.face_normal[3 * b + 2]};
+    }
+    std::vector<dev::CtrlPressureFace> pf(npf);
+    for (std::size_t i = 0; i < npf; ++i) {
+      GSK_CHECK(t.press_face[i] >= 0 && t.press_face[i] < nb_ && t.press_owner[i] >= 0 && t.press_owner[i] < nc_ &&
+                    t.press_slot0[i] >= 0 && t.press_nslots[i] >= 0 &&
+                    static_cast<std::size_t>(t.press_slot0[i]) + t.press_nslots[i] <= nsl,
+                "pressure-face table out of range");
+      pf[i] = dev::CtrlPressureFace{t.press_face[i], t.press_owner[i], t.press_slot0[i], t.press_nslots[i], t.press_eps2[i],
+                                    t.press_to_face[3 * i], t.press_to_face[3 * i + 1], t.press_to_face[3 * i + 2]};
+    }
+    std::vector<dev::CtrlSlot> slots(nsl);
+    for (std::size_t k = 0; k < nsl; ++k) {
+      GSK_CHECK(t.slot_cell[k] >= -1 && t.slot_cell[k] < nc_ &&
+                    (t.slot_cell[k] >= 0 || (t.slot_bface[k] >= 0 && t.slot_bface[k] < nb_)),
+                "stencil slot out of range");
+      slots[k] = dev::CtrlSlot{t.slot_cell[k], t.slot_bface[k], t.slot_periodic[k], 0,
+                               t.slot_lsq[3 * k], t.slot_lsq[3 * k + 1], t.slot_lsq[3 * k + 2],
+                               t.slot_offset[3 * k], t.slot_offset[3 * k + 1], t.slot_offset[3 * k + 2]};
+    }
+    ctl_faces_.upload(faces.data(), nb, stream_);
+    ctl_pf_.upload(pf.data(), npf, stream_);
+    ctl_pf_index_.upload(t.press_face.data(), npf, stream_);
+    ctl_slots_.upload(slots.data(), nsl, stream_);
+    ctl_volume_.upload(t.volume.data(), t.volume.size(), stream_);
+    if (!t.wall_unit.empty()) {
+      GSK_CHECK(t.wall_unit.size() == 10 * nb, "wall unit sums do not match the boundary faces");
+      ctl_wall_unit_.upload(t.wall_unit.data(), t.wall_unit.size(), stream_);
+    }
+    n_pf_ = static_cast<int>(npf);
+    ctl_blocks_ = std::max(1, std::min(1024, (nc_ + dev::kCtlBlock - 1) / dev::kCtlBlock));
+    ctl_part_.alloc(static_cast<std::size_t>(ctl_blocks_) * std::max(dev::kCtlMassParts, dev::kCtlStatParts));
+    prev_moments_.alloc(moments_.n);
+    ctl_gamma_ = t.gamma;
+    ctl_u_floor_ = t.u_floor;
+    ctl_relax_ = t.relaxation;
+    ctl_limiter_ = t.limiter;
+    ctl_cell_mode_ = t.cell_reconstruction;
+    ctl_closed_ = t.closed_domain;
+    HIPCHK(hipStreamSynchronize(stream_));  // the host tables above are released on return
+    ctl_ready_ = true;
+    return true;
+  }
+
+  void control_prepare(ControlStep& st) override {
+    GSK_CHECK(ctl_ready_, "the device prepare has no tables");
+    GSK_CHECK(st.n_faces == static_cast<std::size_t>(nb_) && (nb_ == 0 || (st.inflow && st.face_state && st.flux_out &&
+                                                                          st.flux_in)),
+              "boundary state does not match the boundary faces");
+    GSK_CHECK(st.n_patches > 0 && st.patches && st.patch_flow, "the device prepare needs the patches");
+    const auto nb = static_cast<std::size_t>(nb_), np = st.n_patches;
+    std::vector<dev::CtrlPatch> pp(np);
+    for (std::size_t i = 0; i < np; ++i) {
+      const ControlPatch& c = st.patches[i];
+      GSK_CHECK(c.first >= 0 && c.n >= 0 && c.first + c.n <= nb_, "patch face range out of range");
+      pp[i] = dev::CtrlPatch{c.kind, c.inlet, c.zero_tangential, c.first, c.n, c.p, c.T, c.accommodation};
+    }
+    GSK_CHECK(ctl_max_patch_ < static_cast<int>(np), "boundary face patch out of range");
+    // one device record: control slots | states (6 per face) | face states (5) | F+ (5) | F- (5) |
+    // patch flows | per-face flows
+    const std::size_t o_state = dev::kCtlSize, o_fs = o_state + 6 * nb, o_fo = o_fs + 5 * nb, o_fi = o_fo + 5 * nb,
+                      o_pf = o_fi + 5 * nb, o_ff = o_pf + np, total = o_ff + nb;
+    if (ctl_out_.n != total) ctl_out_.alloc(total);
+    ctl_host_.assign(o_pf, 0.0);
+    ctl_host_[dev::kCtlTarget] = st.mass_target;
+    ctl_host_[dev::kCtlHasTarget] = st.has_mass_target ? 1.0 : 0.0;
+    if (nb > 0) {
+      std::memcpy(&ctl_host_[o_state], st.inflow, nb * sizeof(gsk_inflow_state));
+      std::memcpy(&ctl_host_[o_fs], st.face_state, 5 * nb * sizeof(double));
+    }
+    ctl_patches_.upload(pp.data(), np, stream_);
+    HIPCHK(hipMemcpyAsync(ctl_out_.p, ctl_host_.data(), o_fo * sizeof(double), hipMemcpyHostToDevice, stream_));
+    double* ctrl = ctl_out_.p;
+    double* state = ctrl + o_state;
+    // moments, positivity, mass, closed-domain rescaling, epsilon and the volume means
+    launch_moments();
+    begin(kOther);
+    dev::k_ctrl_mass<<<ctl_blocks_, dev::kCtlBlock, 0, stream_>>>(moments_.p, ctl_volume_.p, nc_, ctl_part_.p);
+    dev::k_ctrl_mass_reduce<<<1, 64, 0, stream_>>>(ctl_part_.p, ctl_blocks_, moments_.p, ctl_closed_ ? 1 : 0, ctrl);
+    if (ctl_closed_) {
+      dev::k_ctrl_scale_f<F><<<4096, 256, 0, stream_>>>(f_.p, static_cast<long long>(f_.n), ctrl);
+      dev::k_ctrl_scale_moments<<<(nc_ + 255) / 256, 256, 0, stream_>>>(moments_.p, nc_, ctrl);
+    }
+    dev::k_ctrl_stats<<<ctl_blocks_, dev::kCtlBlock, 0, stream_>>>(moments_.p, st.has_prev ? prev_moments_.p : nullptr,
+                                                                   ctl_volume_.p, nc_, ctl_part_.p);
+    dev::k_ctrl_stats_reduce<<<1, 64, 0, stream_>>>(ctl_part_.p, ctl_blocks_, st.has_prev ? 1 : 0, ctl_u_floor_, ctrl);
+    HIPCHK(hipGetLastError());
+    end();
+    download(ctl_host_.data(), ctrl, static_cast<std::size_t>(dev::kCtlSize));
+    st.bad = static_cast<long long>(ctl_host_[dev::kCtlBad]);
+    st.mass = ctl_host_[dev::kCtlMass];
+    st.scale = ctl_host_[dev::kCtlScale];
+    st.has_mass_target = ctl_host_[dev::kCtlHasTarget] != 0.0;
+    st.mass_target = ctl_host_[dev::kCtlTarget];
+    for (int k = 0; k < 3; ++k) st.eps[k] = ctl_host_[dev::kCtlEpsRho + k];
+    for (int k = 0; k < 3; ++k) st.q_mean[k] = ctl_host_[dev::kCtlQ + k];
+    for (int k = 0; k < 6; ++k) st.sigma_mean[k] = ctl_host_[dev::kCtlSigma + k];
+    if (st.bad > 0) {  // the host throws before the cell parameters, as the host prepare does
+      st.first_bad = static_cast<int>(ctl_host_[dev::kCtlFirstBad]);
+      st.bad_rho = ctl_host_[dev::kCtlBadRho];
+      st.bad_T = ctl_host_[dev::kCtlBadT];
+      return;
+    }
+    update_cell_params();
+    // boundary rounds
+    const unsigned fb = static_cast<unsigned>((nb + 255) / 256);
+    begin(kBoundary);
+    if (nb > 0 && st.seed_walls)
+      dev::k_ctrl_seed_walls<<<fb, 256, 0, stream_>>>(ctl_faces_.p, ctl_patches_.p, nb_, moments_.p, state);
+    for (long long round = 0; round < st.rounds && nb > 0; ++round) {
+      if (n_pf_ > 0) {
+        dev::k_ctrl_pressure<<<static_cast<unsigned>((n_pf_ + 63) / 64), 64, 0, stream_>>>(
+            ctl_pf_.p, n_pf_, ctl_slots_.p, ctl_faces_.p, ctl_patches_.p, moments_.p, ctrl + o_fs,
+            (round > 0 || st.has_face_state) ? 1 : 0, ctl_limiter_ ? 1 : 0, ctl_cell_mode_ ? 1 : 0, ctl_gamma_, state);
+        if (discrete_eq_)
+          dev::k_inflow_correct<<<static_cast<unsigned>(std::min(n_pf_, 65535)), dev::kBlock, 0, stream_>>>(
+              nv_, xd_.p, yd_.p, zd_.p, weight_, wqd_.p, view().deq, inflow_tol_, state, ctl_pf_index_.p, n_pf_);
+      }
+      dev::k_ctrl_inflow_table<F><<<fb, 256, 0, stream_>>>(state, nb_, inflow_.p, wall_scale_.p);
+      launch_subset_sums(nullptr, nb);
+      dev::k_ctrl_absorb<<<fb, 256, 0, stream_>>>(bsums_.p, ctl_faces_.p, ctl_patches_.p, ctl_wall_unit_.p, nb_,
+                                                  ctl_relax_, state, ctrl + o_fo, ctrl + o_fi, ctrl + o_fs,
+                                                  ctrl + o_ff, ctrl);
+      dev::k_ctrl_patch_flow<<<static_cast<unsigned>((np + 63) / 64), 64, 0, stream_>>>(
+          ctrl + o_ff, ctl_patches_.p, static_cast<int>(np), ctrl + o_pf);
+      dev::k_ctrl_inflow_table<F><<<fb, 256, 0, stream_>>>(state, nb_, inflow_.p, wall_scale_.p);
+    }
+    HIPCHK(hipGetLastError());
+    end();
+    ctl_host_.resize(o_ff);
+    download(ctl_host_.data(), ctrl, o_ff);
+    st.maxwell_negative = ctl_host_[dev::kCtlMaxwellNegative] != 0.0;
+    st.maxwell_no_incoming = ctl_host_[dev::kCtlMaxwellNoIncoming] != 0.0;
+    if (nb > 0) {
+      std::memcpy(st.inflow, &ctl_host_[o_state], nb * sizeof(gsk_inflow_state));
+      std::memcpy(st.face_state, &ctl_host_[o_fs], 5 * nb * sizeof(double));
+      std::memcpy(st.flux_out, &ctl_host_[o_fo], 5 * nb * sizeof(double));
+      std::memcpy(st.flux_in, &ctl_host_[o_fi], 5 * nb * sizeof(double));
+    }
+    std::memcpy(st.patch_flow, &ctl_host_[o_pf], np * sizeof(double));
+    wall_scale_host_.resize(nb);
+    for (std::size_t b = 0; b < nb; ++b) wall_scale_host_[b] = st.inflow[b].scale;
+  }
+
+  void download_moments(double* out, std::size_t n) override {
+    GSK_CHECK(n == moments_.n, "moments output length differs from 15 x cells");
+    download(out, moments_.p, n);
+  }
+  void keep_previous_moments() override {
+    if (prev_moments_.n != moments_.n) prev_moments_.alloc(moments_.n);
+    HIPCHK(hipMemcpyAsync(prev_moments_.p, moments_.p, moments_.bytes(), hipMemcpyDeviceToDevice, stream_));
+  }
+  void set_previous_moments(const double* m, std::size_t n) override {
+    GSK_CHECK(n == moments_.n, "previous moments length differs from 15 x cells");
+    prev_moments_.upload(m, n, stream_);
+    HIPCHK(hipStreamSynchronize(stream_));
+  }
+  void previous_moments(double* out, std::size_t n) override {
+    GSK_CHECK(n == moments_.n && prev_moments_.n == n, "no previous moments on the device");
+    download(out, prev_moments_.p, n);
   }
 
   // ---- adjoint operators (kinetic_adjoint.hpp), FP64 ----
@@ -2633,6 +2819,17 @@ class HipBackend final : public Backend {
   DBuf<double> wall_unit0_, wall_scale_;             // unit-density sums s[0] of Maxwell walls; FP64 scales
   std::vector<int> wall_faces_host_, wall_kind_host_;
   std::vector<double> wall_unit0_host_, wall_scale_host_;
+  // the device prepare (control_setup / control_prepare)
+  bool ctl_ready_ = false, ctl_limiter_ = true, ctl_cell_mode_ = false, ctl_closed_ = false;
+  int n_pf_ = 0, ctl_blocks_ = 1, ctl_max_patch_ = -1;
+  double ctl_gamma_ = 5.0 / 3.0, ctl_u_floor_ = 0.0, ctl_relax_ = 1.0;
+  DBuf<dev::CtrlFace> ctl_faces_;
+  DBuf<dev::CtrlPatch> ctl_patches_;
+  DBuf<dev::CtrlPressureFace> ctl_pf_;
+  DBuf<dev::CtrlSlot> ctl_slots_;
+  DBuf<int> ctl_pf_index_;
+  DBuf<double> ctl_volume_, ctl_wall_unit_, ctl_part_, ctl_out_, prev_moments_;
+  std::vector<double> ctl_host_;
   DBuf<double> bsum_part_, inflow_stage_;
   int bsum_chunks_ = 1;
   std::vector<int> vel_list_host_, dir_off_;

@@ -296,6 +296,15 @@ int gsk_solver_set_newton_tolerances(gsk_solver* sv, double cell_tol, double inf
 /* inflow / boundary-sum rounds per prepare ([kinetic].boundary_iterations); the pressure-inlet
  * reconstruction reads the face states of the previous round */
 int gsk_solver_set_boundary_iterations(gsk_solver* sv, int64_t n);
+/* Where each CIS iteration's prepare runs (no counterpart in the reference, whose
+ * KineticSolver::prepare runs on the host, src/kinetic/kinetic_solver.cpp:365-452): on != 0 (the
+ * default where the backend offers it) keeps the moments, the positivity check, eq. (59) epsilon,
+ * closed-domain rescaling, pressure-face states and the boundary rounds on the backend's device and
+ * exchanges only the boundary state and the iteration's scalars with the host; 0 runs them on the
+ * host. While limiter branches are recorded or followed the host runs them either way.
+ * gsk_solver_device_prepare: *on_device = 1 when the last prepare ran on the device. */
+int gsk_solver_set_device_prepare(gsk_solver* sv, int32_t on);
+int gsk_solver_device_prepare(gsk_solver* sv, int32_t* on_device);
 /* on != 0: the next gsk_solver_residual_device records, at the current f, the branch every limiter
  * takes (the limited gradients' extremes and clamps, and the pressure-face reconstruction's), and
  * later evaluations follow that record, so that R is smooth around the recorded state and

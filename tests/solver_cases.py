@@ -48,22 +48,30 @@ CASES = {
 }
 
 
-def run_case(g, oracle_mod, mg, directory, name):
+def run_case(g, oracle_mod, mg, directory, name, device_prepare=None):
+    """The case on the CPU oracle backend (device_prepare None), or on the HIP backend with each
+    iteration's prepare on the device (True) or on the host (False)."""
     make, iterations, update_pressure = CASES[name]
     d = os.path.join(directory, name)
     os.makedirs(d, exist_ok=True)
     setup = g.Setup(make(mg, d))
-    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=g.BackendOps)
-    sv = g.Solver(setup, cb.ops)
+    cb = oracle_mod.CpuBackend(setup, "oracle", ops_type=g.BackendOps) if device_prepare is None else None
+    sv = g.Solver(setup, cb.ops) if cb is not None else g.Solver(setup)
+    if device_prepare is not None:
+        sv.set_device_prepare(device_prepare)
     sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
-    hist = []
+    hist, where = [], []
     for _ in range(iterations):
         p = sv.prepare()
+        where.append(sv.prepared_on_device())
         s = sv.sweep(update_pressure)
         hist.append([p["epsilon"] if p["has_epsilon"] else 0.0, p["total_mass"], p["mass_scale"], s["residual"]])
     fp, fm, fs = sv.boundary_fluxes()
     out = {"moments": sv.moments(), "f_plus": fp, "f_minus": fm, "face_state": fs, "flows": sv.patch_mass_flow(),
            "history": np.array(hist)}
     sv.close()
-    cb.close()
+    if cb is not None:
+        cb.close()
+    if device_prepare is not None:
+        assert where == [device_prepare] * iterations, where
     return out
