@@ -409,14 +409,19 @@ __global__ void k_ctrl_absorb(const double* __restrict__ sums, const CtrlFace* _
   face_flow[b] = flux_out[static_cast<i64>(b) * 5] + flux_in[static_cast<i64>(b) * 5];
 }
 
-// patch mass flows: each patch's faces in order
-__global__ void k_ctrl_patch_flow(const double* __restrict__ face_flow, const CtrlPatch* __restrict__ patches,
-                                  int n_patches, double* __restrict__ flow) {
-  const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+// patch mass flows: one block per patch, its faces strided over the threads, then a fixed-order
+// block sum
+__global__ __launch_bounds__(kCtlBlock) void k_ctrl_patch_flow(const double* __restrict__ face_flow,
+                                                               const CtrlPatch* __restrict__ patches, int n_patches,
+                                                               double* __restrict__ flow) {
+  __shared__ double lds[kCtlBlock / 64];
+  const int pi = blockIdx.x;
   if (pi >= n_patches) return;
-  double s = 0.0;
-  for (int b = patches[pi].first; b < patches[pi].first + patches[pi].n; ++b) s += face_flow[b];
-  flow[pi] = s;
+  double s[1] = {0.0};
+  const int b0 = patches[pi].first, b1 = patches[pi].first + patches[pi].n;
+  for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += kCtlBlock) s[0] += face_flow[b];
+  ctrl_block_sum<1>(s, lds);
+  if (threadIdx.x == 0) flow[pi] = s[0];
 }
 
 }  // namespace gsk::dev

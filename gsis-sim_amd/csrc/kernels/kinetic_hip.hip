@@ -2185,8 +2185,8 @@ class HipBackend final : public Backend {
       dev::k_ctrl_absorb<<<fb, 256, 0, stream_>>>(bsums_.p, ctl_faces_.p, ctl_patches_.p, ctl_wall_unit_.p, nb_,
                                                   ctl_relax_, state, ctrl + o_fo, ctrl + o_fi, ctrl + o_fs,
                                                   ctrl + o_ff, ctrl);
-      dev::k_ctrl_patch_flow<<<static_cast<unsigned>((np + 63) / 64), 64, 0, stream_>>>(
-          ctrl + o_ff, ctl_patches_.p, static_cast<int>(np), ctrl + o_pf);
+      dev::k_ctrl_patch_flow<<<static_cast<unsigned>(np), dev::kCtlBlock, 0, stream_>>>(ctrl + o_ff, ctl_patches_.p,
+                                                                                        static_cast<int>(np), ctrl + o_pf);
       dev::k_ctrl_inflow_table<F><<<fb, 256, 0, stream_>>>(state, nb_, inflow_.p, wall_scale_.p);
     }
     HIPCHK(hipGetLastError());
