@@ -7,6 +7,14 @@ must cost at most 3x the forward solve. The four difference solves take about 7 
 values are constants here, from the run recorded in
 profiles/sensitivity_tube40k_24cubed_maxwell_alpha0.8_adjoint.json (bench.py --sensitivity
 --alpha 0.8 --fd-check); that run measured 1.2e-6 and 4.9e-7 and a cost of 0.78x.
+
+The sensitivity configuration with FP32 f: the forward solve in the mixed form (FP32 f, matrix-core
+moment sums) converged to residual 1e-6, its f handed to the FP64 solver, whose adjoint solve at that
+state gives both derivatives. They must lie within 1e-4 of the same central differences and within
+1e-4 of the FP64 forward's adjoint values (profiles/sensitivity_tube40k_24cubed_maxwell_alpha0.8_adjoint.json;
+the mixed run in profiles/sensitivity_tube40k_24cubed_maxwell_alpha0.8_mixed_forward_adjoint.json
+measured 5.7e-5 / 4.8e-5 from the differences and 5.5e-5 / 4.8e-5 from the FP64 values, at 1.9x the
+mixed forward solve), and the adjoint must cost at most 3x that forward solve.
 """
 
 from __future__ import annotations
@@ -19,6 +27,8 @@ pytestmark = pytest.mark.gpu
 FD_DDELTA = 0.01899277790812226
 FD_DALPHA = -0.44044936697851256
 MDOT = 0.6974269243754333  # the forward solve's Mdot in that run
+ADJ_DDELTA = 0.018992800954406655  # the FP64 forward's adjoint values in that run
+ADJ_DALPHA = -0.44044958486717845
 
 
 @pytest.mark.timeout(600)
@@ -50,5 +60,48 @@ def test_tube40k_alpha08_adjoint_within_1e4_of_central_differences(gsk, meshgen,
     sv.close()
     assert abs(float(mdot.detach()) - MDOT) <= 1e-8 * MDOT
     assert e_d <= 1e-4 and e_a <= 1e-4
+    assert prob.stats["adjoint_true_relative_residual"] <= 1e-7
+    assert cost <= 3.0
+
+
+@pytest.mark.timeout(600)
+def test_tube40k_alpha08_fp32_forward_adjoint_within_1e4(gsk, meshgen, tmp_path):
+    import gsis_sim_amd.sensitivity as sens
+
+    delta, alpha = 10.0, 0.8
+    cases = {}
+    for name, kinetic in (("tube_maxwell", {"precision": "fp64"}),
+                          ("tube_maxwell_mixed", {"precision": "fp32", "moment_sums": "mfma"})):
+        cases[name] = meshgen.tube_case(str(tmp_path), n_core=16, n_ring=10, n_axial=45, n_vel=24, vrange=6.0,
+                                        delta=delta, accommodation=alpha, kinetic=kinetic, name=name)
+    setup = gsk.Setup(cases["tube_maxwell"])
+    sv = gsk.Solver(setup)
+    setup32 = gsk.Setup(cases["tube_maxwell_mixed"])
+    sv32 = gsk.Solver(setup32)
+    sv32.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    prob = sens.SteadyProblem(sv, inlet=0, mdot_factor=setup.mass_flow_factor(), wall_patch=2, delta_rp=delta,
+                              alpha=alpha)
+    prob.use_forward_solver(sv32)
+    prob.log = lambda m: print(f"[gpu] tube40k alpha 0.8 FP32 forward {m}", flush=True)
+    prob.polish = 120
+    prob.restart = 20
+    d_t = torch.tensor(delta, dtype=torch.float64, requires_grad=True)
+    a_t = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
+    sv.initialize_uniform(1.0, (0.0, 0.0, 0.0), 1.0)
+    mdot = sens.converged_mdot(d_t, a_t, prob, residual_target=1e-6)
+    fwd = prob.stats["forward_seconds"]
+    sv32.close()
+    mdot.backward()
+    cost = prob.stats["sensitivity_seconds"] / fwd
+    g_d, g_a = float(d_t.grad), float(a_t.grad)
+    e_d, e_a = abs(g_d - FD_DDELTA) / abs(FD_DDELTA), abs(g_a - FD_DALPHA) / abs(FD_DALPHA)
+    r_d, r_a = abs(g_d - ADJ_DDELTA) / abs(ADJ_DDELTA), abs(g_a - ADJ_DALPHA) / abs(ADJ_DALPHA)
+    print(f"[gpu] tube40k alpha 0.8 FP32 forward: Mdot {float(mdot.detach()):.10f}; dMdot/ddelta {g_d:.8e} "
+          f"(vs differences {e_d:.1e}, vs FP64 adjoint {r_d:.1e}), dMdot/dalpha {g_a:.8e} ({e_a:.1e}, {r_a:.1e}); "
+          f"adjoint {prob.stats['sensitivity_seconds']:.1f} s = {cost:.2f} x forward", flush=True)
+    sv.close()
+    assert abs(float(mdot.detach()) - MDOT) <= 2e-4 * MDOT
+    assert e_d <= 1e-4 and e_a <= 1e-4
+    assert r_d <= 1e-4 and r_a <= 1e-4
     assert prob.stats["adjoint_true_relative_residual"] <= 1e-7
     assert cost <= 3.0
