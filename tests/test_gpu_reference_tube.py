@@ -3,8 +3,9 @@ reference's own results: its coarse tube-to-vacuum case (tests/golden/tube_hex_c
 reference's mesh from its own generator; 14,464 hexahedra, 7 RCB blocks, 30 x 24 x 24 velocities)
 and its converged CIS runs of that case (tests/golden/tube_coarse_reference_results.csv, from
 cases/tube_vacuum/kinetic/results/param_study*.csv and phase5/results/cis_study_coarse.csv). These
-numbers were computed by the reference (its FP32 CUDA backend) and do not pass through this build's
-orchestration or oracle, so they pin the boundary handling that the HIP-vs-oracle tests share.
+numbers were computed by the reference (its FP32 CUDA backend) and do not pass through this
+library's orchestration or its oracle, so they pin the boundary handling that the HIP-vs-oracle
+tests share.
 
 The reference stopped its runs at epsilon < 1e-5, where the inlet and outlet flows still differ
 (by 1.3% at delta_rp = 1, by up to 30% at delta_rp = 10) and the iterates depend on the sweep settings
